@@ -1,0 +1,73 @@
+// madam_internal.h -- device-side node layout shared by the HIP kernels and
+// the host engine.  Not part of the C ABI (include/madam_bound.h is).
+//
+// One B&B node's ADMM state lives in HBM as a NodeView: FP64 column-major
+// n x n matrices with leading dimension n, the cut rows of B in two CSR
+// forms, and the per-node scalars of the ADMM loop.  A batch is an array of
+// NodeViews; every kernel takes the array and a node index from blockIdx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace madam {
+
+// Cut row encoding.  A cut c with vertex tuple v[0..k-1] and sign vector b
+// has the row B(X)_c = scale_c * sum_{a<b} sgn_ab * X[v_a, v_b] with
+// sgn_ab = -b_a b_b and scale_c = 1, 1/2, 1/3 for k = 3, 5, 7
+// (operators.c:85-148).  Each (a,b) term is one CSR entry holding the lower
+// linear index lo = max(v_a,v_b) + n*min(v_a,v_b), stored as lo for
+// sgn = +1 and ~lo for sgn = -1, in the reference's pair order
+// (0,1),(0,2),...,(1,2),...  The transpose groups the same entries by
+// "active pair" (a distinct vertex pair touched by at least one cut): each
+// active pair lists (cut, sign) in increasing cut order, stored as c or ~c.
+struct NodeView {
+  int n;       // matrix order (PP->n)
+  int m;       // cuts (triangles + pentagonals + heptagonals, in that order)
+  int nact;    // active pairs
+  int pad0;
+
+  double *L, *X, *Q, *W, *V, *ev, *y;  // n*n (L, X, Q, W, V), n (ev, y)
+  double *S1, *S2;                     // n*n scratch
+  double *evp;                         // n: kept eigenvalues of the latest projection (0 if dropped)
+
+  const int *cptr;      // m+1
+  const int *cent;      // cptr[m]: lo or ~lo
+  const int *cact;      // cptr[m]: active-pair id of the entry
+  const double *cscale; // m
+  const int *aptr;      // nact+1
+  const int *aent;      // aptr[nact]: c or ~c
+  const int *apair;     // nact: lower linear index of the active pair
+  const int *p2a;       // n*n: active-pair id of lower index lo, or -1
+
+  double *t, *s, *u, *rhs;             // m
+  double *cg_r, *cg_p, *cg_q, *cg_w;   // m, m, m, nact
+
+  // per-node ADMM scalars
+  double sigma, tol, scaleSigma, normL;
+  double f, lam, lam_margin, off_last;
+  double err, err_p, err_d, gap;
+  double primal, dual;
+  double sigma_last;   // sigma used by the latest projection (X = sigma_last * X_+)
+  int itermax, nbit, done, M;
+  int cg_iters, cg_total, sweeps_last, sweeps_total;
+  int eig_warm;   // V holds a usable basis from the previous projection
+  int pad1;
+};
+
+// Launch helpers implemented in madam_device.hip.  All take the device array
+// of nodes and the batch size, and run on `stream`.
+void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
+void launch_cg(NodeView* nodes, int nb, double rtol, int maxit, hipStream_t stream);
+void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
+void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
+                   int ignore_done, size_t lds_limit, hipStream_t stream);
+void launch_admm_post(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+void launch_safe_bound(NodeView* nodes, int nb, hipStream_t stream);
+void launch_op_B(const NodeView* nodes, int nb, const double* Xin, double* yout, int mmax,
+                 hipStream_t stream);
+void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout, int nmax,
+                  hipStream_t stream);
+void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+
+}  // namespace madam
