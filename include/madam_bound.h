@@ -100,6 +100,16 @@ void madam_close(void);
 /* Last error message (static storage). */
 const char *madam_last_error(void);
 
+/* Per-phase GPU timing (no reference counterpart; the reference only takes
+ * MPI_Wtime stamps, admm.c:674).  Phases, in order: 0 prep (y, rhs),
+ * 1 cg (BB^T+I solve), 2 w (duals + W), 3 eig (projection eigen-solve),
+ * 4 post (X_+, residuals, stop test), 5 safe (safe bound), 6 ipm,
+ * 7 trisep, 8 sa, 9 gw.  ms[p] = summed event time of all launches of
+ * phase p, count[p] = number of timed launch groups. */
+#define MADAM_NPHASE 10
+int madam_profile(int on);
+int madam_profile_read(int nphase, double *ms, long *count, int reset);
+
 /* ADMM_solver (admm.c:49-771) for k independent nodes, all on the GPU in
  * one batch: per node the same iteration, stop test and safe bound as the
  * reference.  opts may be null (defaults). */

@@ -207,7 +207,42 @@ struct Slot {
   }
 };
 
+// Optional per-phase GPU timing with HIP events on the engine's stream.
+enum Phase { PH_PREP, PH_CG, PH_W, PH_EIG, PH_POST, PH_SAFE, PH_IPM, PH_TRISEP, PH_SA, PH_GW,
+             PH_COUNT };
+
+struct PhaseTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double ms[PH_COUNT] = {0};
+  long count[PH_COUNT] = {0};
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // resolve recorded pairs; the stream must be synchronised by the caller
+  void flush() {
+    for (auto& pr : pending) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, pr.second.first, pr.second.second);
+      ms[pr.first] += t;
+      count[pr.first] += 1;
+      pool.push_back(pr.second.first);
+      pool.push_back(pr.second.second);
+    }
+    pending.clear();
+  }
+};
+
 struct Engine {
+  PhaseTimer timer;
   int device = -1;
   hipStream_t stream = nullptr;
   size_t lds_limit = 0;
@@ -218,6 +253,20 @@ struct Engine {
 };
 
 Engine* g_eng = nullptr;
+
+// Run `call` (kernel launches on stream st) bracketed by events when timing.
+#define TIMED(phase, st, call)                                          \
+  do {                                                                  \
+    if (g_eng->timer.on) {                                              \
+      hipEvent_t a_ = g_eng->timer.get(), b_ = g_eng->timer.get();      \
+      (void)hipEventRecord(a_, st);                                     \
+      call;                                                             \
+      (void)hipEventRecord(b_, st);                                     \
+      g_eng->timer.pending.push_back({phase, {a_, b_}});                \
+    } else {                                                            \
+      call;                                                             \
+    }                                                                   \
+  } while (0)
 
 int ensure_open() {
   if (g_eng) return MADAM_OK;
@@ -314,6 +363,30 @@ int sync_views_down(int k) {
 extern "C" {
 
 const char* madam_last_error(void) { return g_err; }
+
+int madam_profile(int on) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  g_eng->timer.on = on != 0;
+  return MADAM_OK;
+}
+
+int madam_profile_read(int nphase, double* ms, long* count, int reset) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g_eng->stream));
+  g_eng->timer.flush();
+  for (int p = 0; p < nphase && p < PH_COUNT; ++p) {
+    if (ms) ms[p] = g_eng->timer.ms[p];
+    if (count) count[p] = g_eng->timer.count[p];
+  }
+  if (reset)
+    for (int p = 0; p < PH_COUNT; ++p) {
+      g_eng->timer.ms[p] = 0;
+      g_eng->timer.count[p] = 0;
+    }
+  return MADAM_OK;
+}
 
 int madam_open(int device) {
   if (g_eng) return MADAM_OK;
@@ -412,24 +485,29 @@ int madam_admm_batch(int k, madam_admm_job* jobs, const madam_opts* opts_in) {
     }
     launched += chunk;
     for (int it = 0; it < chunk; ++it) {
-      madam::launch_admm_prep(E.views.p, k, mmax, st);
-      madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st);
-      madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st);
-      madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st);
-      madam::launch_admm_post(E.views.p, k, nmax, st);
+      TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
+      TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
+      TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
+      TIMED(PH_EIG, st,
+            madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st));
+      TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, st));
     }
     HIP_TRY(hipGetLastError());
     if ((rc = sync_views_down(k))) return rc;
+    E.timer.flush();
     bool all = true;
     for (int b = 0; b < k; ++b) all = all && E.hviews[b].done;
     if (all) break;
   }
   // ---- safe bound (admm.c:724-763)
-  madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
-  madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st);
-  madam::launch_safe_bound(E.views.p, k, st);
+  TIMED(PH_SAFE, st, {
+    madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
+    madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st);
+    madam::launch_safe_bound(E.views.p, k, st);
+  });
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(k))) return rc;
+  E.timer.flush();
 
   // ---- results
   std::vector<double> evh, Vh;
