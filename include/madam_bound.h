@@ -131,6 +131,86 @@ int madam_project_sdp(int n, const double *W, double *X, double *Q, double *Z, i
  * returns an upper bound lam >= lambda_max(A) within |E|_F + 1e-12 |A|_F. */
 int madam_lambda_max(int n, const double *A, double *lam);
 
+/* getViolated_TriangleInequalities (cutting_planes.c:48-127): the at most
+ * K triangle rows with B_c(X) - 1 > thresh of largest value, in the
+ * reference's enumeration order (type, then i > j > k ascending); *maxval is
+ * the largest row value over all 4 C(n,3) rows.  Same set as the reference
+ * except where rows tie at the K-th value (then earlier rows win). */
+int madam_tri_separate(int n, const double *X, int K, double thresh, madam_tri *list, int *size,
+                       double *maxval);
+
+/* qap_simulated_annealing (qap_simulated_annealing.c:12-206) for nruns
+ * independent runs on X: run r uses the sign pattern of types[r] (H = ee^T
+ * with the first types[r]-1 entries of e negated) and its own random stream
+ * (seed, stream_ids[r]); cost[r] and perms[r*k .. r*k+k-1] are the best
+ * value and k-tuple, draws[r] the number of random values consumed. */
+int madam_qap_sa(int n, const double *X, int k, int nruns, const int *types,
+                 const long long *stream_ids, unsigned long long seed, double *cost, int *perms,
+                 long long *draws);
+
+/* ipm_mc_pk (ipm_mc_pk.c:28-347): basic SDP max <L,X>, diag(X) = e, by the
+ * predictor-corrector interior-point method; X, y, Z = Diag(y) - L and the
+ * dual value *phi once phi - psi <= 1e-4.  MADAM_E_EIG where the reference
+ * exits on a failed Cholesky factorisation. */
+int madam_ipm(int n, const double *L, double *X, double *y, double *Z, double *phi, int *iters);
+
+/* GW_heuristic + mc_1opt (heuristic.c:24-201) on a given factor Z (n x M):
+ * num hyperplanes from stream (seed, stream), hyperplane h has components
+ * u - 1/2 with u = (splitmix64(key + j) >> 11) * 2^-53,
+ * key = splitmix64(seed ^ splitmix64((stream << 24) + h)).  cuts[h*n + i]
+ * gets the 1-opt cut in +-1 form, values[h] = z'^T L z' with z' the cut
+ * whose last (homogenising) entry is set to +1 (heuristic.c:76-84). */
+int madam_gw_rounding(int n, const double *L, const double *Z, int M, int num,
+                      unsigned long long seed, long long stream, double *values,
+                      signed char *cuts);
+
+/* Layout-identical to BabNode (admm.h:147-155). */
+typedef struct {
+    int xfixed[MADAM_NMAX];
+    int solX[MADAM_NMAX];           /* BabSolution sol */
+    double fracsol[MADAM_NMAX];
+    int level;
+    double upper_bound;
+} madam_node;
+
+/* What the reference keeps in process globals for bounding and branching:
+ * the instance (SP), the parameters, the incumbent (heap.c:8-9), diff
+ * (global_var.h:13), plus this library's random-stream state and counters. */
+typedef struct {
+    int N;                          /* SP->n: vertices; vertex N-1 is fixed to side 0 */
+    const double *SPL;              /* SP->L, N x N (process_input.c:235-299) */
+    madam_params params;            /* in/out: Pent/Hepta_Trials grow (bounding.c:184-187) */
+    double LB;                      /* incumbent value (Bab_LBGet) */
+    int best_x[MADAM_NMAX];         /* incumbent 0/1 vector (BabSol->X) */
+    double diff;                    /* basic SDP bound - cut bound at the root */
+    unsigned long long seed;        /* random streams of SA and GW rounding */
+    long long stream_ctr;           /* in/out: next stream id */
+    int projection;                 /* 0: FP64 eigen-decomposition projection */
+    int batch;                      /* nodes bounded together by madam_bab */
+    long long nodes;                /* out: evaluated nodes */
+    long long admm_iters, admm_calls, ipm_iters;
+    long long open_nodes;           /* out: nodes left when max_nodes stopped the search */
+    double root_bound;
+    double seconds_ipm, seconds_admm, seconds_sep, seconds_heur;  /* host wall time */
+} madam_bab_ctx;
+
+/* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n
+ * subproblem matrix (n = free vertices + 1), *fixedvalue the constant. */
+int madam_create_subproblem(int N, const double *SPL, const int *xfixed, const int *solX,
+                            double *PPL, int *n_out, double *fixedvalue);
+
+/* SDPbound (bounding.c:25) for k nodes at once: ranks[q] = 0 marks the
+ * root (bounding.c:111, 241); bounds[q] = the node's upper bound, also
+ * written to nodes[q]->upper_bound, with fracsol filled.  The incumbent in
+ * ctx is updated by the heuristic as updateSolution does. */
+int madam_sdpbound_batch(madam_bab_ctx *ctx, int k, madam_node *const *nodes, const int *ranks,
+                         double *bounds);
+
+/* Single-process branch and bound (bab_functions.c:39-69, 226-334,
+ * heap.c) bounding ctx->batch nodes per step.  max_nodes <= 0: no limit.
+ * *optimum = ctx->LB at the end (the proven optimum when open_nodes = 0). */
+int madam_bab(madam_bab_ctx *ctx, long long max_nodes, double *optimum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -242,3 +242,200 @@ def ADMM_solver(L, X, y, Q, sigma, tol, tri=(), pent=(), hept=(), itermax=2000,
     return ADMM_solver_batch([dict(L=L, X=X, y=y, Q=Q, tri=tri, pent=pent, hept=hept,
                                    sigma=sigma, tol=tol, itermax=itermax,
                                    scaleSigma=scaleSigma)], **opts)[0]
+
+
+def tri_separate(X, K, thresh=1e-3):
+    """getViolated_TriangleInequalities (cutting_planes.c:48): (max value,
+    [(type, i, j, k, value), ...]) in enumeration order."""
+    X = _f64(X)
+    n = X.shape[0]
+    lst = (Tri * max(1, K))()
+    size = C.c_int(0)
+    mx = C.c_double(0)
+    _check(lib().madam_tri_separate(C.c_int(n), X.ctypes.data_as(_PD), C.c_int(K),
+                                    C.c_double(thresh), lst, C.byref(size), C.byref(mx)))
+    return mx.value, [(lst[q].type, lst[q].i, lst[q].j, lst[q].k, lst[q].value)
+                      for q in range(size.value)]
+
+
+def qap_sa(X, k, types, stream_ids, seed):
+    """Independent qap_simulated_annealing runs (qap_simulated_annealing.c:12)."""
+    X = _f64(X)
+    n = X.shape[0]
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    sid = np.ascontiguousarray(stream_ids, dtype=np.int64)
+    nr = len(types)
+    cost = np.zeros(nr)
+    perms = np.zeros((nr, k), np.int32)
+    draws = np.zeros(nr, np.int64)
+    _check(lib().madam_qap_sa(C.c_int(n), X.ctypes.data_as(_PD), C.c_int(k), C.c_int(nr),
+                              types.ctypes.data_as(_PI), sid.ctypes.data_as(C.POINTER(C.c_longlong)),
+                              C.c_ulonglong(seed), cost.ctypes.data_as(_PD), perms.ctypes.data_as(_PI),
+                              draws.ctypes.data_as(C.POINTER(C.c_longlong))))
+    return cost, perms, draws
+
+
+def rand31_stream(seed, stream_id, count):
+    """The 31-bit random stream of SA run `stream_id` (splitmix64 based),
+    reproduced on the host: value_t = splitmix64(base + t) >> 33 with
+    base = splitmix64(seed ^ splitmix64(stream_id))."""
+    def sm(z):
+        z = (z + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+    with np.errstate(over="ignore"):
+        base = sm(np.uint64(seed) ^ sm(np.uint64(stream_id)))
+        t = np.arange(count, dtype=np.uint64)
+        return (sm(base + t) >> np.uint64(33)).astype(np.int64).astype(np.int32)
+
+
+# ---------------------------------------------------------------- bounding
+
+NMAX = 1024
+
+
+class Params(C.Structure):
+    """madam_params == Parameters (admm.h:57-89)."""
+    _fields_ = [
+        ("sigma0", C.c_double), ("scaleSigma", C.c_double), ("ADMM_itermax", C.c_int),
+        ("tol0", C.c_double), ("scaleTol", C.c_double), ("minTol", C.c_double),
+        ("triag_iter", C.c_int), ("pent_iter", C.c_int), ("hept_iter", C.c_int),
+        ("max_outer_iter", C.c_int), ("extra_iter", C.c_int), ("violated_TriIneq", C.c_double),
+        ("TriIneq", C.c_int), ("adjust_TriIneq", C.c_int), ("PentIneq", C.c_int),
+        ("HeptaIneq", C.c_int), ("Pent_Trials", C.c_int), ("Hepta_Trials", C.c_int),
+        ("include_Pent", C.c_int), ("include_Hepta", C.c_int), ("root", C.c_int),
+        ("use_diff", C.c_int), ("branchingStrategy", C.c_int),
+    ]
+
+
+def default_params(**over):
+    """The reference's params file (params:1-23)."""
+    p = Params(sigma0=1.0, scaleSigma=1.001, ADMM_itermax=2000, tol0=5e-2, scaleTol=0.6,
+               minTol=1e-3, triag_iter=3, pent_iter=3, hept_iter=3, max_outer_iter=12,
+               extra_iter=5, violated_TriIneq=1e-3, TriIneq=5000, adjust_TriIneq=1,
+               PentIneq=5000, HeptaIneq=5000, Pent_Trials=60, Hepta_Trials=50, include_Pent=1,
+               include_Hepta=1, root=0, use_diff=1, branchingStrategy=1)
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class Node(C.Structure):
+    """madam_node == BabNode (admm.h:147-155)."""
+    _fields_ = [("xfixed", C.c_int * NMAX), ("solX", C.c_int * NMAX),
+                ("fracsol", C.c_double * NMAX), ("level", C.c_int), ("upper_bound", C.c_double)]
+
+
+class BabCtx(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("SPL", _PD), ("params", Params), ("LB", C.c_double),
+        ("best_x", C.c_int * NMAX), ("diff", C.c_double), ("seed", C.c_ulonglong),
+        ("stream_ctr", C.c_longlong), ("projection", C.c_int), ("batch", C.c_int),
+        ("nodes", C.c_longlong), ("admm_iters", C.c_longlong), ("admm_calls", C.c_longlong),
+        ("ipm_iters", C.c_longlong), ("open_nodes", C.c_longlong), ("root_bound", C.c_double),
+        ("seconds_ipm", C.c_double), ("seconds_admm", C.c_double), ("seconds_sep", C.c_double),
+        ("seconds_heur", C.c_double),
+    ]
+
+
+def create_subproblem(SPL, xfixed, solX):
+    """createSubproblem + getFixedValue (evaluate.c:39-139) -> (PP->L, fixedvalue)."""
+    SPL = _f64(SPL)
+    N = SPL.shape[0]
+    xf = np.zeros(N, np.int32)
+    sx = np.zeros(N, np.int32)
+    xf[:N - 1] = xfixed
+    sx[:N - 1] = solX
+    out = np.zeros(N * N)
+    n = C.c_int(0)
+    fv = C.c_double(0)
+    _check(lib().madam_create_subproblem(C.c_int(N), SPL.ctypes.data_as(_PD), xf.ctypes.data_as(_PI),
+                                         sx.ctypes.data_as(_PI), out.ctypes.data_as(_PD), C.byref(n),
+                                         C.byref(fv)))
+    k = n.value
+    return np.asfortranarray(out[:k * k].reshape(k, k, order="F")), fv.value
+
+
+def ipm(L):
+    """ipm_mc_pk (ipm_mc_pk.c:28): X, y, Z, phi, iterations."""
+    L = _f64(L)
+    n = L.shape[0]
+    X = np.zeros((n, n), order="F")
+    Z = np.zeros((n, n), order="F")
+    y = np.zeros(n)
+    phi = C.c_double(0)
+    its = C.c_int(0)
+    _check(lib().madam_ipm(C.c_int(n), L.ctypes.data_as(_PD), X.ctypes.data_as(_PD), y.ctypes.data_as(_PD),
+                           Z.ctypes.data_as(_PD), C.byref(phi), C.byref(its)))
+    return X, y, Z, phi.value, its.value
+
+
+def gw_planes(seed, stream, num, M):
+    """The hyperplanes madam_gw_rounding draws (see include/madam_bound.h)."""
+    def sm(z):
+        z = (z + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+    out = np.zeros((num, M))
+    with np.errstate(over="ignore"):
+        for h in range(num):
+            key = sm(np.uint64(seed) ^ sm((np.uint64(stream) << np.uint64(24)) + np.uint64(h)))
+            j = np.arange(M, dtype=np.uint64)
+            u = (sm(key + j) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            out[h] = u - 0.5
+    return out
+
+
+def gw_rounding(L, Z, num, seed=2020, stream=0):
+    """GW_heuristic + mc_1opt (heuristic.c:24-201) on factor Z: (values, cuts)."""
+    L = _f64(L)
+    n = L.shape[0]
+    Z = _f64(Z)
+    M = Z.shape[1] if Z.ndim == 2 else 0
+    vals = np.zeros(num)
+    cuts = np.zeros((num, n), np.int8)
+    _check(lib().madam_gw_rounding(C.c_int(n), L.ctypes.data_as(_PD), Z.ctypes.data_as(_PD), C.c_int(M),
+                                   C.c_int(num), C.c_ulonglong(seed), C.c_longlong(stream),
+                                   vals.ctypes.data_as(_PD), cuts.ctypes.data_as(C.POINTER(C.c_byte))))
+    return vals, cuts
+
+
+class Bounder:
+    """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
+
+    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0):
+        self.SPL = _f64(SPL)
+        self.ctx = BabCtx()
+        self.ctx.N = self.SPL.shape[0]
+        self.ctx.SPL = self.SPL.ctypes.data_as(_PD)
+        self.ctx.params = params if params is not None else default_params()
+        if self.ctx.params.adjust_TriIneq:
+            self.ctx.params.TriIneq = self.ctx.N * 10
+        self.ctx.seed = seed
+        self.ctx.batch = batch
+        self.ctx.LB = LB
+
+    def sdpbound(self, nodes, ranks=None):
+        k = len(nodes)
+        arr = (C.POINTER(Node) * k)(*[C.pointer(x) for x in nodes])
+        rk = np.ascontiguousarray(ranks if ranks is not None else [1] * k, dtype=np.int32)
+        out = np.zeros(k)
+        _check(lib().madam_sdpbound_batch(C.byref(self.ctx), C.c_int(k), arr, rk.ctypes.data_as(_PI),
+                                          out.ctypes.data_as(_PD)))
+        return out
+
+    def node(self, xfixed=None, solX=None):
+        x = Node()
+        if xfixed is not None:
+            for i, v in enumerate(xfixed):
+                x.xfixed[i] = int(v)
+            for i, v in enumerate(solX):
+                x.solX[i] = int(v)
+        return x
+
+    def bab(self, max_nodes=0):
+        opt = C.c_double(0)
+        _check(lib().madam_bab(C.byref(self.ctx), C.c_longlong(max_nodes), C.byref(opt)))
+        return opt.value

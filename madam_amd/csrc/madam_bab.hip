@@ -1,0 +1,835 @@
+// madam_bab.hip -- node evaluation and branch-and-bound driver of
+// libmadam_bound.so: createSubproblem / getFixedValue (evaluate.c:39-139),
+// SDPbound's cutting-plane loop (bounding.c:25-250) for a batch of nodes in
+// lockstep, and a single-process B&B (bab_functions.c:39-69, 226-334 and
+// heap.c without MPI).
+//
+// Every numerical step runs on the GPU: the interior-point warm start
+// (k_ipm), the ADMM calls (admm_loop), triangle separation, the simulated
+// annealing of pentagonal/heptagonal separation and the GW rounding.  The
+// host keeps what the reference keeps in process globals: the cut lists
+// (purged, deduplicated and appended here exactly as cutting_planes.c does),
+// the incumbent, diff and the parameters that grow on give-up.
+#include <chrono>
+#include <unordered_set>
+
+#include "madam_engine.h"
+
+namespace madam {
+void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t st);
+}
+
+using namespace mh;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// createSubproblem (evaluate.c:39-116).  Returns PP->n.
+int create_subproblem(int N, const double* SPL, const int* xfixed, const int* solX,
+                      std::vector<double>& PPL) {
+  const int bs = N - 1;
+  int nfree = 0;
+  for (int i = 0; i < bs; ++i) nfree += !xfixed[i];
+  const int n = nfree + 1;
+  PPL.assign((size_t)n * n, 0.0);
+  std::vector<double> frow(std::max(nfree, 1), 0.0);
+  int col = 0;
+  for (int i = 0; i < bs; ++i) {
+    if (xfixed[i]) {
+      if (solX[i] == 1) {
+        int f = 0;
+        for (int j = 0; j < bs; ++j)
+          if (!xfixed[j]) frow[f++] += SPL[j + (size_t)i * N];
+      }
+      continue;
+    }
+    int r = 0;
+    double rs = 0.0;
+    for (int j = 0; j < bs; ++j) {
+      if (xfixed[j]) continue;
+      const double v = SPL[j + (size_t)i * N];
+      PPL[r + (size_t)col * n] = v;
+      rs += v;
+      ++r;
+    }
+    PPL[(n - 1) + (size_t)col * n] = rs;
+    ++col;
+  }
+  for (int i = 0; i < n - 1; ++i) PPL[i + (size_t)(n - 1) * n] = PPL[(n - 1) + (size_t)i * n];
+  double corner = 0.0;
+  for (int i = 0; i < n - 1; ++i) {
+    PPL[i + (size_t)i * n] += 2 * frow[i];
+    PPL[(n - 1) + (size_t)i * n] += 2 * frow[i];
+    PPL[i + (size_t)(n - 1) * n] += 2 * frow[i];
+    corner += PPL[i + (size_t)(n - 1) * n];
+  }
+  PPL[(n - 1) + (size_t)(n - 1) * n] = corner;
+  for (auto& v : PPL) v *= 0.25;
+  return n;
+}
+
+// getFixedValue (evaluate.c:124-139).
+double fixed_value(int N, const double* SPL, const int* xfixed, const int* solX) {
+  double v = 0.0;
+  for (int i = 0; i < N - 1; ++i)
+    for (int j = 0; j < N - 1; ++j)
+      if (xfixed[i] && xfixed[j]) v += SPL[j + (size_t)i * N] * solX[i] * solX[j];
+  return v;
+}
+
+double evaluate_solution(int N, const double* SPL, const int* sol) {
+  double v = 0.0;
+  for (int i = 0; i < N - 1; ++i)
+    if (sol[i])
+      for (int j = 0; j < N - 1; ++j) v += SPL[j + (size_t)i * N] * sol[i] * sol[j];
+  return v;
+}
+
+// Host state of one node during SDPbound (the reference's per-call locals
+// and the cut arrays of global_var.h:33-42).
+struct Run {
+  madam_node* node = nullptr;
+  int rank = 1, slot = 0, n = 0;
+  double fixedvalue = 0, basic_bound = 0, bound = 0, oldf = 0, f = 0;
+  double sigma = 1, tol = 0.05, viol3 = 0, viol5 = 0;
+  int count = 0, nbit = 0, done = 0, giveup = 0, prune = 0;
+  std::vector<double> PPL;
+  std::vector<madam_tri> tri;
+  std::vector<madam_pent> pent;
+  std::vector<madam_hept> hept;
+  std::vector<double> s;  // cut duals after the last ADMM call, layout [tri|pent|hept]
+};
+
+struct Ctx {
+  madam_bab_ctx* c;
+  Engine& E;
+};
+
+void store_fracsol(Run& r, int N, const std::vector<double>& lastcol) {
+  int idx = 0;
+  for (int i = 0; i < N - 1; ++i) {
+    if (r.node->xfixed[i])
+      r.node->fracsol[i] = (double)r.node->solX[i];
+    else
+      r.node->fracsol[i] = 0.5 * (lastcol[idx++] + 1.0);
+  }
+}
+
+int fetch_fracsol(std::vector<Run*>& runs, int N) {
+  Engine& E = *g_eng;
+  std::vector<double> col;
+  for (Run* r : runs) {
+    col.resize(r->n);
+    int rc = down(col.data(), E.slots[r->slot].X.p + (size_t)(r->n - 1) * r->n, r->n, E.stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(E.stream));
+    store_fracsol(*r, N, col);
+  }
+  return MADAM_OK;
+}
+
+// GW rounding on the runs' slots; updates the incumbent like updateSolution
+// (bab_functions.c:114-134).  zero_planes: the call right after the IPM.
+int heuristic(madam_bab_ctx* c, std::vector<Run*>& runs, bool zero_planes) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)runs.size();
+  if (k == 0) return MADAM_OK;
+  int rc;
+  const int num = zero_planes ? 1 : c->N;
+  std::vector<madam::GwNode> g(k);
+  std::vector<NodeView> hv(k);
+  int nmax = 2;
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    Slot& sl = E.slots[r.slot];
+    if ((rc = sl.size_gw(r.n, num))) return rc;
+    hv[q] = E.hviews[r.slot];
+    g[q].num = num;
+    g[q].zero_planes = zero_planes ? 1 : 0;
+    g[q].seed = c->seed;
+    g[q].stream = c->stream_ctr++;
+    g[q].fixedvalue = r.fixedvalue;
+    g[q].val = sl.gwval.p;
+    g[q].zall = sl.gwz.p;
+    g[q].best_z = sl.gwbz.p;
+    g[q].best_val = sl.gwbest.p;
+    nmax = std::max(nmax, r.n);
+  }
+  if ((rc = E.views.ensure(k)) || (rc = E.gwnodes.ensure(k))) return rc;
+  if ((rc = up(E.views.p, hv.data(), k, st)) || (rc = up(E.gwnodes.p, g.data(), k, st))) return rc;
+  TIMED(PH_GW, st, madam::launch_gw(E.views.p, E.gwnodes.p, k, nmax, num, st));
+  HIP_TRY(hipGetLastError());
+  std::vector<signed char> z;
+  std::vector<int> sol(c->N);
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    Slot& sl = E.slots[r.slot];
+    double bv = 0;
+    z.resize(r.n);
+    if ((rc = down(&bv, sl.gwbest.p, 1, st)) || (rc = down(z.data(), sl.gwbz.p, r.n, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    int idx = 0;
+    for (int i = 0; i < c->N - 1; ++i)
+      sol[i] = r.node->xfixed[i] ? r.node->solX[i] : (z[idx++] + 1) / 2;
+    if (bv > c->LB) {
+      // confirm on the original objective before taking it as the incumbent
+      const double v = evaluate_solution(c->N, c->SPL, sol.data());
+      if (v > c->LB) {
+        c->LB = v;
+        for (int i = 0; i < c->N - 1; ++i) c->best_x[i] = sol[i];
+      }
+    }
+  }
+  E.timer.flush();
+  return MADAM_OK;
+}
+
+// Top-K list semantics of getViolated_Pentagonal/HeptagonalInequalities
+// (cutting_planes.c:247-303, 444-501) applied to SA results in run order.
+template <class Cut, int KK>
+void kgon_list(const std::vector<double>& cost, const std::vector<int>& perms,
+               const std::vector<int>& types, int K, std::vector<Cut>& list, double& best) {
+  list.clear();
+  int slot = 0;
+  double slotval = -1e9;
+  best = 1e9;
+  for (size_t r = 0; r < cost.size(); ++r) {
+    const double val = cost[r];
+    if (val < best) best = val;
+    if (!(val < 1.0)) continue;
+    Cut cut{};
+    cut.type = types[r];
+    cut.value = val;
+    for (int a = 0; a < KK; ++a) cut.perm[a] = perms[r * KK + a];
+    if ((int)list.size() < K) {
+      if (list.empty() || val > slotval) {
+        slotval = val;
+        slot = (int)list.size();
+      }
+      list.push_back(cut);
+    } else if (val < slotval) {
+      list[slot] = cut;
+      slotval = 1e9;
+      for (int q = 0; q < (int)list.size(); ++q)
+        if (list[q].value > slotval) {
+          slotval = list[q].value;
+          slot = q;
+        }
+    }
+  }
+}
+
+// Purge (dual < 1e-5) and append non-duplicates (cutting_planes.c:132-214,
+// 316-409, 512-605).  Duplicates are checked against the survivors only.
+template <class Cut, class Same>
+void purge_append(std::vector<Cut>& cuts, const double* dual, const std::vector<Cut>& list,
+                  Same same) {
+  std::vector<Cut> keep;
+  keep.reserve(cuts.size() + list.size());
+  for (size_t c = 0; c < cuts.size(); ++c) {
+    Cut x = cuts[c];
+    x.y = dual[c];
+    if (x.y < 1e-5) continue;
+    keep.push_back(x);
+  }
+  const size_t survivors = keep.size();
+  for (const Cut& cand : list) {
+    if (keep.size() == 50000) break;
+    bool dup = false;
+    for (size_t c = 0; c < survivors && !dup; ++c) dup = same(keep[c], cand);
+    if (dup) continue;
+    Cut x = cand;
+    x.y = 0.0;
+    keep.push_back(x);
+  }
+  cuts.swap(keep);
+}
+
+int separate_tri(madam_bab_ctx* c, std::vector<Run*>& runs) {
+  const int k = (int)runs.size();
+  if (k == 0) return MADAM_OK;
+  std::vector<int> K(k, c->params.TriIneq), slots(k);
+  for (int q = 0; q < k; ++q) slots[q] = runs[q]->slot;
+  std::vector<std::vector<madam_tri>> lists;
+  std::vector<double> mx;
+  int rc = tri_separate_slots(slots, K.data(), c->params.violated_TriIneq, lists, mx);
+  if (rc) return rc;
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    std::vector<double> dual(r.tri.size());
+    for (size_t i = 0; i < r.tri.size(); ++i) dual[i] = r.s[i];
+    // dedup by key among the survivors
+    std::vector<madam_tri> keep;
+    for (size_t i = 0; i < r.tri.size(); ++i) {
+      madam_tri x = r.tri[i];
+      x.y = dual[i];
+      if (x.y >= 1e-5) keep.push_back(x);
+    }
+    std::unordered_set<uint64_t> have;
+    auto key = [](const madam_tri& t) {
+      return ((uint64_t)t.type << 48) | ((uint64_t)t.i << 32) | ((uint64_t)t.j << 16) | (uint64_t)t.k;
+    };
+    for (auto& t : keep) have.insert(key(t));
+    for (auto& cand : lists[q]) {
+      if (keep.size() == 50000) break;
+      if (have.count(key(cand))) continue;
+      madam_tri x = cand;
+      x.y = 0.0;
+      keep.push_back(x);
+    }
+    r.tri.swap(keep);
+    r.viol3 = mx[q];
+  }
+  return MADAM_OK;
+}
+
+int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
+                  const std::vector<int>& offsets) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)runs.size();
+  if (k == 0) return MADAM_OK;
+  const int KK = hept ? 7 : 5, ntypes = hept ? 4 : 3;
+  const int trials = hept ? c->params.Hepta_Trials : c->params.Pent_Trials;
+  const int per = trials * ntypes;
+  const int total = per * k;
+  if (total == 0) return MADAM_OK;
+  int rc;
+  if ((rc = E.sumx.ensure(k)) || (rc = E.xptrs.ensure(k)) || (rc = E.nvals.ensure(k)) ||
+      (rc = E.sajobs.ensure(total)) || (rc = E.saout.ensure(total)))
+    return rc;
+  std::vector<const double*> xp(k);
+  std::vector<int> ns(k);
+  int nmax = 2;
+  for (int q = 0; q < k; ++q) {
+    xp[q] = E.slots[runs[q]->slot].X.p;
+    ns[q] = runs[q]->n;
+    nmax = std::max(nmax, ns[q]);
+  }
+  if ((rc = up(E.xptrs.p, xp.data(), k, st)) || (rc = up(E.nvals.p, ns.data(), k, st))) return rc;
+  madam::launch_sumx(E.xptrs.p, E.nvals.p, E.sumx.p, k, st);
+  std::vector<madam::SaJob> jobs(total);
+  std::vector<int> types(per);
+  for (int t = 0; t < trials; ++t)
+    for (int ty = 1; ty <= ntypes; ++ty) types[t * ntypes + ty - 1] = ty;
+  for (int q = 0; q < k; ++q)
+    for (int r = 0; r < per; ++r) {
+      madam::SaJob& J = jobs[(size_t)q * per + r];
+      J.X = xp[q];
+      J.sumX = E.sumx.p + q;
+      J.n = ns[q];
+      J.k = KK;
+      J.neg = types[r] - 1;
+      J.nstride = madam::sa_nstride(nmax);
+      J.seed = c->seed;
+      J.stream_id = c->stream_ctr++;
+    }
+  if ((rc = up(E.sajobs.p, jobs.data(), total, st))) return rc;
+  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, total, nmax, E.saout.p, st));
+  HIP_TRY(hipGetLastError());
+  std::vector<madam::SaOut> out(total);
+  if ((rc = down(out.data(), E.saout.p, total, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  E.timer.flush();
+  std::vector<double> cost(per);
+  std::vector<int> perms((size_t)per * KK);
+  for (int q = 0; q < k; ++q) {
+    Run& R = *runs[q];
+    for (int r = 0; r < per; ++r) {
+      const madam::SaOut& o = out[(size_t)q * per + r];
+      cost[r] = o.cost;
+      for (int a = 0; a < KK; ++a) perms[(size_t)r * KK + a] = o.perm[a];
+    }
+    const double* dual = R.s.data() + offsets[q];
+    double best;
+    if (hept) {
+      std::vector<madam_hept> list;
+      kgon_list<madam_hept, 7>(cost, perms, types, c->params.HeptaIneq, list, best);
+      purge_append(R.hept, dual, list, [](const madam_hept& a, const madam_hept& b) {
+        if (a.type != b.type) return false;
+        for (int x = 0; x < 7; ++x)
+          if (a.perm[x] != b.perm[x]) return false;
+        return true;
+      });
+    } else {
+      std::vector<madam_pent> list;
+      kgon_list<madam_pent, 5>(cost, perms, types, c->params.PentIneq, list, best);
+      purge_append(R.pent, dual, list, [](const madam_pent& a, const madam_pent& b) {
+        if (a.type != b.type) return false;
+        for (int x = 0; x < 5; ++x)
+          if (a.perm[x] != b.perm[x]) return false;
+        return true;
+      });
+      R.viol5 = best;
+    }
+  }
+  return MADAM_OK;
+}
+
+// One round of ADMM_solver calls for the runs (admm.c:49-771).
+int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)runs.size();
+  if (k == 0) return MADAM_OK;
+  int rc;
+  std::vector<CutSet> cs(k);
+  std::vector<NodeView> hv(k);
+  int nmax = 2, mmax = 1;
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    if ((rc = build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
+                           (int)r.hept.size(), r.hept.data(), cs[q])))
+      return rc;
+    Slot& sl = E.slots[r.slot];
+    if ((rc = sl.size_for(r.n, cs[q])) || (rc = upload_cuts(sl, cs[q], st))) return rc;
+    if (cs[q].m > 0) {
+      if ((rc = up(sl.s.p, cs[q].y0.data(), cs[q].m, st)) ||
+          (rc = up(sl.t.p, cs[q].y0.data(), cs[q].m, st)))
+        return rc;
+      HIP_TRY(hipMemsetAsync(sl.u.p, 0, cs[q].m * sizeof(double), st));
+    }
+    NodeView v = E.hviews[r.slot];
+    NodeView fresh;
+    bind_view(fresh, sl, r.n, cs[q]);
+    // keep the node's scalars, take the new cut structure pointers
+    fresh.normL = v.normL;
+    fresh.eig_warm = 0;
+    fresh.sigma = r.sigma;
+    fresh.tol = r.tol;
+    fresh.scaleSigma = c->params.scaleSigma;
+    fresh.itermax = c->params.ADMM_itermax;
+    hv[q] = fresh;
+    nmax = std::max(nmax, r.n);
+    mmax = std::max(mmax, cs[q].m);
+  }
+  madam_opts o;
+  o.cg_rtol = 1e-12;
+  o.cg_maxit = 1000;
+  o.jacobi_sweeps = 40;
+  o.check_every = 16;
+  if ((rc = admm_loop(hv, o, nmax, mmax))) return rc;
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    E.hviews[r.slot] = hv[q];
+    r.sigma = hv[q].sigma;
+    r.nbit = hv[q].nbit;
+    r.f = hv[q].f;
+    c->admm_iters += hv[q].nbit;
+    c->admm_calls += 1;
+    r.s.resize(cs[q].m);
+    if (cs[q].m > 0 && (rc = down(r.s.data(), E.slots[r.slot].s.p, cs[q].m, st))) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return MADAM_OK;
+}
+
+}  // namespace
+
+
+namespace {
+
+int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)runs.size();
+  int rc, nmax = 2;
+  std::vector<NodeView> hv(k);
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    Slot& sl = E.slots[r.slot];
+    CutSet cs0;
+    if ((rc = build_cutset(r.n, 0, nullptr, 0, nullptr, 0, nullptr, cs0)) ||
+        (rc = sl.size_for(r.n, cs0)) || (rc = sl.size_ipm(r.n)))
+      return rc;
+    if ((rc = up(sl.L.p, r.PPL.data(), r.PPL.size(), st))) return rc;
+    bind_view(hv[q], sl, r.n, cs0);
+    double fro = 0.0;
+    for (double v : r.PPL) fro += v * v;
+    hv[q].normL = sqrt(fro);
+    nmax = std::max(nmax, r.n);
+  }
+  if ((rc = E.views.ensure(k)) || (rc = up(E.views.p, hv.data(), k, st))) return rc;
+  TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, E.lds_limit, st));
+  HIP_TRY(hipGetLastError());
+  if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  E.timer.flush();
+  for (int q = 0; q < k; ++q) {
+    Run& r = *runs[q];
+    if (hv[q].ipm_status) {
+      set_err("interior-point method failed (status %d) on a node of order %d", hv[q].ipm_status,
+              r.n);
+      return MADAM_E_EIG;
+    }
+    E.hviews[r.slot] = hv[q];
+    r.f = hv[q].f;
+    c->ipm_iters += hv[q].ipm_iters;
+  }
+  return MADAM_OK;
+}
+
+// SDPbound (bounding.c:25-250) for k nodes in lockstep.
+int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int* ranks,
+                   double* bounds) {
+  Engine& E = *g_eng;
+  madam_params& P = c->params;
+  int rc;
+  if ((rc = ensure_slots(k))) return rc;
+  std::vector<Run> runs(k);
+  std::vector<Run*> all;
+  for (int q = 0; q < k; ++q) {
+    Run& r = runs[q];
+    r.node = nodes[q];
+    r.rank = ranks ? ranks[q] : 1;
+    r.slot = q;
+    r.n = create_subproblem(c->N, c->SPL, r.node->xfixed, r.node->solX, r.PPL);
+    r.fixedvalue = fixed_value(c->N, c->SPL, r.node->xfixed, r.node->solX);
+    r.sigma = P.sigma0;
+    r.tol = P.tol0;
+    all.push_back(&r);
+  }
+  double t0 = now_s();
+  if ((rc = ipm_round(c, all))) return rc;
+  c->seconds_ipm += now_s() - t0;
+  for (Run* r : all) r->basic_bound = r->f + r->fixedvalue;
+  if ((rc = fetch_fracsol(all, c->N))) return rc;
+  t0 = now_s();
+  if ((rc = heuristic(c, all, true))) return rc;
+  c->seconds_heur += now_s() - t0;
+  std::vector<Run*> act;
+  for (Run* r : all) {
+    r->bound = r->f + r->fixedvalue;
+    if (r->bound < c->LB + 1.0) {
+      r->done = 2;  // pruned before the loop: END without diff
+      continue;
+    }
+    if (P.use_diff && r->rank != 0 && r->bound > c->LB + c->diff + 1.0) {
+      r->done = 2;
+      continue;
+    }
+    act.push_back(r);
+  }
+  t0 = now_s();
+  for (Run* r : act) r->s.clear();
+  if ((rc = separate_tri(c, act))) return rc;
+  c->seconds_sep += now_s() - t0;
+
+  while (!act.empty()) {
+    for (Run* r : act) {
+      ++r->count;
+      r->oldf = r->f;
+    }
+    t0 = now_s();
+    if ((rc = admm_round(c, act))) return rc;
+    c->seconds_admm += now_s() - t0;
+    std::vector<Run*> heur;
+    for (Run* r : act) {
+      r->tol *= P.scaleTol;
+      if (r->tol < P.minTol) r->tol = P.minTol;
+      r->bound = r->f + r->fixedvalue;
+      r->prune = r->bound < c->LB + 1.0;
+      if (!r->prune) heur.push_back(r);
+    }
+    t0 = now_s();
+    if ((rc = heuristic(c, heur, false))) return rc;
+    c->seconds_heur += now_s() - t0;
+    std::vector<Run*> sep;
+    for (Run* r : act) {
+      if (!r->prune) r->prune = r->bound < c->LB + 1.0;
+      const double gap = r->bound - c->LB;
+      if (r->count == P.triag_iter + P.pent_iter + P.hept_iter &&
+          gap - 1.0 > (r->oldf - r->f) * (P.max_outer_iter - r->count))
+        r->giveup = 1;
+      if (r->count == P.max_outer_iter && gap - 1.0 > (r->oldf - r->f) * P.extra_iter)
+        r->giveup = 1;
+      if (r->count == P.max_outer_iter + P.extra_iter) r->giveup = 1;
+      if (r->rank != 0 && r->giveup && !r->prune) {
+        P.Pent_Trials += 60;
+        P.Hepta_Trials += 50;
+      }
+      if (!r->prune && !r->giveup) sep.push_back(r);
+    }
+    t0 = now_s();
+    if (!sep.empty()) {
+      std::vector<int> triag(sep.size()), penta(sep.size());
+      for (size_t q = 0; q < sep.size(); ++q) {
+        triag[q] = (int)sep[q]->tri.size();
+        penta[q] = (int)sep[q]->pent.size();
+      }
+      if ((rc = separate_tri(c, sep))) return rc;
+      std::vector<Run*> ps, hs;
+      std::vector<int> poff, hoff;
+      for (size_t q = 0; q < sep.size(); ++q) {
+        Run* r = sep[q];
+        if (P.include_Pent && (r->count >= P.triag_iter || r->viol3 < 0.2)) {
+          ps.push_back(r);
+          poff.push_back(triag[q]);
+        }
+      }
+      if ((rc = separate_kgon(c, ps, false, poff))) return rc;
+      for (size_t q = 0; q < sep.size(); ++q) {
+        Run* r = sep[q];
+        if (P.include_Hepta && ((r->count >= P.triag_iter + P.pent_iter) ||
+                                (r->viol3 < 0.2 && (1 - r->viol5 < 0.4)))) {
+          hs.push_back(r);
+          hoff.push_back(triag[q] + penta[q]);
+        }
+      }
+      if ((rc = separate_kgon(c, hs, true, hoff))) return rc;
+    }
+    c->seconds_sep += now_s() - t0;
+    std::vector<Run*> next;
+    for (Run* r : act) {
+      r->done = r->prune || r->giveup || (r->nbit >= P.ADMM_itermax);
+      if (!r->done) next.push_back(r);
+    }
+    if ((rc = fetch_fracsol(act, c->N))) return rc;
+    act.swap(next);
+  }
+  for (Run* r : all) {
+    if (r->done != 2) {
+      r->bound = r->f + r->fixedvalue;
+      if (r->rank == 0) c->diff = r->basic_bound - r->bound;
+    }
+    bounds[r->slot] = r->bound;
+    r->node->upper_bound = r->bound;
+  }
+  c->nodes += k;
+  return MADAM_OK;
+}
+
+// getBranchingVariable (bab_functions.c:389-423).
+int branch_var(const madam_bab_ctx* c, const madam_node* node) {
+  int ic = -1;
+  if (c->params.branchingStrategy == 0) {
+    double mx = -1e9;
+    for (int i = 0; i < c->N - 1; ++i)
+      if (!node->xfixed[i] && fabs(0.5 - node->fracsol[i]) > mx) {
+        ic = i;
+        mx = fabs(0.5 - node->fracsol[i]);
+      }
+  } else {
+    double mn = 1e9;
+    for (int i = 0; i < c->N - 1; ++i)
+      if (!node->xfixed[i] && fabs(0.5 - node->fracsol[i]) < mn) {
+        ic = i;
+        mn = fabs(0.5 - node->fracsol[i]);
+      }
+  }
+  return ic;
+}
+
+// Max-heap on upper_bound (heap.c:27-133); children carry their parent's
+// bound (heap.c:143-168 leaves it unset).
+struct Heap {
+  std::vector<madam_node*> d;
+  static bool gt(const madam_node* a, const madam_node* b) { return a->upper_bound > b->upper_bound; }
+  void push(madam_node* x) {
+    d.push_back(x);
+    size_t c = d.size() - 1;
+    while (c > 0) {
+      size_t p = (c - 1) / 2;
+      if (!gt(d[c], d[p])) break;
+      std::swap(d[c], d[p]);
+      c = p;
+    }
+  }
+  madam_node* pop() {
+    madam_node* top = d[0];
+    d[0] = d.back();
+    d.pop_back();
+    size_t c = 0;
+    for (;;) {
+      size_t l = 2 * c + 1, b = c;
+      if (l < d.size() && gt(d[l], d[b])) b = l;
+      if (l + 1 < d.size() && gt(d[l + 1], d[b])) b = l + 1;
+      if (b == c) break;
+      std::swap(d[c], d[b]);
+      c = b;
+    }
+    return top;
+  }
+};
+
+madam_node* child_of(const madam_bab_ctx* c, const madam_node* parent, int ic, int val) {
+  madam_node* x = new madam_node();
+  for (int i = 0; i < c->N - 1; ++i) {
+    x->xfixed[i] = parent->xfixed[i];
+    x->solX[i] = parent->xfixed[i] ? parent->solX[i] : 0;
+  }
+  x->xfixed[ic] = 1;
+  x->solX[ic] = val;
+  x->level = parent->level + 1;
+  x->upper_bound = parent->upper_bound;
+  return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int madam_create_subproblem(int N, const double* SPL, const int* xfixed, const int* solX,
+                            double* PPL, int* n_out, double* fixedvalue) {
+  if (N < 2 || N > MADAM_NMAX || !SPL || !xfixed || !solX || !PPL || !n_out) {
+    set_err("madam_create_subproblem: bad arguments");
+    return MADAM_E_ARG;
+  }
+  std::vector<double> L;
+  const int n = create_subproblem(N, SPL, xfixed, solX, L);
+  std::copy(L.begin(), L.end(), PPL);
+  *n_out = n;
+  if (fixedvalue) *fixedvalue = fixed_value(N, SPL, xfixed, solX);
+  return MADAM_OK;
+}
+
+int madam_sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int* ranks,
+                         double* bounds) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (!c || k <= 0 || !nodes || !bounds || c->N < 3 || c->N > MADAM_NMAX || !c->SPL) {
+    set_err("madam_sdpbound_batch: bad arguments");
+    return MADAM_E_ARG;
+  }
+  return sdpbound_batch(c, k, nodes, ranks, bounds);
+}
+
+int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (!c || c->N < 3 || c->N > MADAM_NMAX || !c->SPL) {
+    set_err("madam_bab: bad arguments");
+    return MADAM_E_ARG;
+  }
+  if (c->params.adjust_TriIneq) c->params.TriIneq = c->N * 10;
+  const int batch = std::max(1, c->batch);
+  madam_node* root = new madam_node();
+  int rank0 = 0;
+  double b0 = 0;
+  if ((rc = sdpbound_batch(c, 1, &root, &rank0, &b0))) {
+    delete root;
+    return rc;
+  }
+  c->root_bound = b0;
+  root->upper_bound = b0;
+  Heap h;
+  if (c->LB + 1.0 < b0 && !c->params.root) {
+    const int ic = branch_var(c, root);
+    h.push(child_of(c, root, ic, 0));
+    h.push(child_of(c, root, ic, 1));
+  }
+  delete root;
+  while (!h.d.empty()) {
+    if (max_nodes > 0 && c->nodes >= max_nodes) break;
+    std::vector<madam_node*> cur;
+    while (!h.d.empty() && (int)cur.size() < batch) {
+      madam_node* x = h.pop();
+      if (c->LB + 1.0 < x->upper_bound)
+        cur.push_back(x);
+      else
+        delete x;  // the parent's bound already prunes it
+    }
+    if (cur.empty()) continue;
+    std::vector<int> ranks(cur.size(), 1);
+    std::vector<double> bs(cur.size());
+    if ((rc = sdpbound_batch(c, (int)cur.size(), cur.data(), ranks.data(), bs.data()))) {
+      for (auto* x : cur) delete x;
+      for (auto* x : h.d) delete x;
+      return rc;
+    }
+    for (size_t q = 0; q < cur.size(); ++q) {
+      madam_node* x = cur[q];
+      if (c->LB + 1.0 < x->upper_bound) {
+        const int ic = branch_var(c, x);
+        if (ic >= 0) {
+          h.push(child_of(c, x, ic, 0));
+          h.push(child_of(c, x, ic, 1));
+        }
+      }
+      delete x;
+    }
+  }
+  c->open_nodes = (long long)h.d.size();
+  for (auto* x : h.d) delete x;
+  if (optimum) *optimum = c->LB;
+  return MADAM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int madam_ipm(int n, const double* L, double* X, double* y, double* Z, double* phi, int* iters) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (n < 2 || n > MADAM_NMAX || !L || !X || !y || !Z || !phi) {
+    set_err("madam_ipm: bad arguments");
+    return MADAM_E_ARG;
+  }
+  if ((rc = ensure_slots(1))) return rc;
+  madam_bab_ctx c{};
+  Run r;
+  r.slot = 0;
+  r.n = n;
+  r.PPL.assign(L, L + (size_t)n * n);
+  std::vector<Run*> runs{&r};
+  if ((rc = ipm_round(&c, runs))) return rc;
+  Engine& E = *g_eng;
+  Slot& sl = E.slots[0];
+  const size_t nn = (size_t)n * n;
+  if ((rc = down(X, sl.X.p, nn, E.stream)) || (rc = down(Z, sl.Q.p, nn, E.stream)) ||
+      (rc = down(y, sl.y.p, (size_t)n, E.stream)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  *phi = r.f;
+  if (iters) *iters = (int)c.ipm_iters;
+  return MADAM_OK;
+}
+
+int madam_gw_rounding(int n, const double* L, const double* Z, int M, int num,
+                      unsigned long long seed, long long stream, double* values,
+                      signed char* cuts) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (n < 2 || n > MADAM_NMAX || M < 0 || M > n || num <= 0 || !L || !values) {
+    set_err("madam_gw_rounding: bad arguments");
+    return MADAM_E_ARG;
+  }
+  if ((rc = ensure_slots(1))) return rc;
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  Slot& sl = E.slots[0];
+  CutSet cs0;
+  if ((rc = build_cutset(n, 0, nullptr, 0, nullptr, 0, nullptr, cs0)) || (rc = sl.size_for(n, cs0)) ||
+      (rc = sl.size_gw(n, num)))
+    return rc;
+  const size_t nn = (size_t)n * n;
+  if ((rc = up(sl.L.p, L, nn, st))) return rc;
+  if (M > 0 && (rc = up(sl.S1.p, Z, (size_t)n * M, st))) return rc;
+  NodeView v;
+  bind_view(v, sl, n, cs0);
+  madam::GwNode g{};
+  g.num = num;
+  g.M = M;
+  g.seed = seed;
+  g.stream = stream;
+  g.val = sl.gwval.p;
+  g.zall = sl.gwz.p;
+  g.best_z = sl.gwbz.p;
+  g.best_val = sl.gwbest.p;
+  if ((rc = E.views.ensure(1)) || (rc = E.gwnodes.ensure(1)) || (rc = up(E.views.p, &v, 1, st)) ||
+      (rc = up(E.gwnodes.p, &g, 1, st)))
+    return rc;
+  madam::launch_gw_given_z(E.views.p, E.gwnodes.p, 1, n, num, st);
+  HIP_TRY(hipGetLastError());
+  if ((rc = down(values, sl.gwval.p, num, st))) return rc;
+  if (cuts && (rc = down(cuts, sl.gwz.p, (size_t)num * n, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  return MADAM_OK;
+}
+
+}  // extern "C"

@@ -10,158 +10,11 @@
 // operations, so that for identical inputs they give the same bits as
 // operators.c (op_B, op_Diag, op_Bt) and the elementwise lines of admm.c.
 #include "madam_internal.h"
+#include "madam_devutil.h"
 
 #include <math.h>
 
 namespace madam {
-
-constexpr int WG = 1024;   // per-node kernels (16 waves)
-constexpr int WGS = 256;   // streaming kernels
-
-// ------------------------------------------------------------------ utils
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Sum of NV values over the workgroup; every thread receives the totals.
-// `red` must hold NV * (WG/64) doubles.  Deterministic order.
-template <int NV>
-__device__ __forceinline__ void block_sums(double (&v)[NV], double* red) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) red[q * 16 + w] = v[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double r = 0.0;
-    for (int i = 0; i < nw; ++i) r += red[q * 16 + i];
-    v[q] = r;
-  }
-}
-
-__device__ __forceinline__ double block_max(double v, double* red) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  v = wave_max(v);
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double r = red[0];
-  for (int i = 1; i < nw; ++i) r = fmax(r, red[i]);
-  return r;
-}
-
-__device__ __forceinline__ int ent_idx(int e) { return e >= 0 ? e : ~e; }
-
-// Packed lower-triangular index (column-major) for an order-n2 matrix.
-__device__ __forceinline__ int pk(int i, int j, int n2) {
-  if (i < j) {
-    int t = i;
-    i = j;
-    j = t;
-  }
-  return i + j * (n2 - 1) - ((j * (j - 1)) >> 1);
-}
-
-// Round-robin (circle method) pairing: round r, pair k, n2 even.
-__device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
-  const int N1 = n2 - 1;
-  if (k == 0) {
-    p = N1;
-    q = r;
-  } else {
-    p = (r + k) % N1;
-    q = (r - k + N1) % N1;
-  }
-}
-
-// f -> (k, l) with k >= l, f = k(k+1)/2 + l.
-__device__ __forceinline__ void tri_decode(int f, int& k, int& l) {
-  int kk = (int)((sqrt(8.0 * f + 1.0) - 1.0) * 0.5);
-  while ((kk + 1) * (kk + 2) / 2 <= f) ++kk;
-  while (kk * (kk + 1) / 2 > f) --kk;
-  k = kk;
-  l = f - kk * (kk + 1) / 2;
-}
-
-// -------------------------------------------------------- tiled FP64 GEMM
-// C(MxN) = op(A) * diag(d) * op(B) over K, one workgroup of WG threads.
-// op(A)[i,k] = TA ? A[k + i*lda] : A[i + k*lda]; op(B)[k,j] = TB ? B[j + k*ldb]
-// : B[k + j*ldb]; d may be null.  `out(i, j, v)` stores a result.  Tiles of
-// 64x64 outputs, 16-deep k slabs staged in LDS (tA, tB: 16*65 doubles each).
-template <bool TA, bool TB, class Out>
-__device__ void wg_gemm(const double* __restrict__ A, int lda, const double* __restrict__ B,
-                        int ldb, const double* __restrict__ d, int M, int N, int K, Out out,
-                        double* tA, double* tB) {
-  const int tid = threadIdx.x;
-  const int ti = tid & 31, tj = tid >> 5;  // 32 x 32 threads, each 2x2 outputs
-  for (int i0 = 0; i0 < M; i0 += 64)
-    for (int j0 = 0; j0 < N; j0 += 64) {
-      double acc00 = 0, acc01 = 0, acc10 = 0, acc11 = 0;
-      for (int k0 = 0; k0 < K; k0 += 16) {
-        {  // A slab: 64 (i) x 16 (k)
-          int i, kk;
-          if (TA) {
-            kk = tid & 15;
-            i = tid >> 4;
-          } else {
-            i = tid & 63;
-            kk = tid >> 6;
-          }
-          const int gi = i0 + i, gk = k0 + kk;
-          double v = 0.0;
-          if (gi < M && gk < K) {
-            v = TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda];
-            if (d) v *= d[gk];
-          }
-          tA[kk * 65 + i] = v;
-        }
-        {  // B slab: 16 (k) x 64 (j)
-          int j, kk;
-          if (TB) {
-            j = tid & 63;
-            kk = tid >> 6;
-          } else {
-            kk = tid & 15;
-            j = tid >> 4;
-          }
-          const int gj = j0 + j, gk = k0 + kk;
-          double v = 0.0;
-          if (gj < N && gk < K) v = TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb];
-          tB[kk * 65 + j] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-          const double a0 = tA[kk * 65 + ti], a1 = tA[kk * 65 + ti + 32];
-          const double b0 = tB[kk * 65 + tj], b1 = tB[kk * 65 + tj + 32];
-          acc00 = fma(a0, b0, acc00);
-          acc01 = fma(a0, b1, acc01);
-          acc10 = fma(a1, b0, acc10);
-          acc11 = fma(a1, b1, acc11);
-        }
-        __syncthreads();
-      }
-      const int r0 = i0 + ti, r1 = i0 + ti + 32, c0 = j0 + tj, c1 = j0 + tj + 32;
-      if (r0 < M && c0 < N) out(r0, c0, acc00);
-      if (r0 < M && c1 < N) out(r0, c1, acc01);
-      if (r1 < M && c0 < N) out(r1, c0, acc10);
-      if (r1 < M && c1 < N) out(r1, c1, acc11);
-    }
-}
 
 // ------------------------------------------------------------ ADMM steps
 
@@ -328,15 +181,20 @@ __global__ __launch_bounds__(WGS) void k_build_w(NodeView* nodes, int use_s) {
 // ------------------------------------------------------ Jacobi eigensolver
 //
 // Cyclic two-sided Jacobi with round-robin parallel ordering, one workgroup
-// per node.  The working matrix A is kept as a packed lower triangle, in LDS
-// when it fits (IN_LDS) and in the node's S2 scratch otherwise.  Warm start:
-// when nd.eig_warm is set, A = V^T W V with V the eigenvectors of the
-// previous projection (nearly diagonal, so one or two sweeps finish it) and
-// V accumulates the new rotations; cold start uses A = W, V = I.
+// per node.  The working matrix A is a packed lower triangle, in LDS when it
+// fits (A_LDS) and in the node's S2 scratch otherwise; the eigenvector
+// matrix V lives in LDS too when A and V both fit (V_LDS), else in HBM.
+// Warm start: when nd.eig_warm is set, A = V^T W V with V the eigenvectors
+// of the previous projection (nearly diagonal, one or two sweeps finish it)
+// and V accumulates the new rotations; cold start uses A = W, V = I.
 // values_only (safe bound) never touches V and always starts cold.
+// Sweeps stop when the off-diagonal Frobenius norm is <= 1e-10 |A|_F.
 //
 // Output: ev[0..n-1] = diag(A) (unsorted), V (unless values_only),
-// off_last = off-diagonal Frobenius norm left, sweeps_last.
+// off_last, sweeps_last; with form_xplus also the projection
+// X_+ = V diag(lambda_+) V^T into S1 (full), evp and M, where lambda_+ keeps
+// eigenvalues > 1e-8 (the dsyevr range (1e-8, VU] of operators.c:536-579;
+// VU = min(|W|_inf, |W|_F) >= lambda_max, so only the lower end matters).
 
 struct PackedOut {
   double* A;
@@ -351,9 +209,14 @@ struct DenseOut {
   __device__ void operator()(int i, int j, double v) const { C[i + (size_t)j * ld] = v; }
 };
 
-template <bool IN_LDS>
+// Dynamic LDS layout of k_jacobi (in doubles): rc, rs, rt [h each], pair
+// table [h doubles = 2h ints], red [64], tA, tB [16*65 each], then A
+// (packed, if A_LDS), then V (n*n, if V_LDS).
+__host__ __device__ inline size_t jac_fixed_doubles(int h) { return 4 * (size_t)h + 64 + 2 * 16 * 65; }
+
+template <bool A_LDS, bool V_LDS>
 __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only, int max_sweeps,
-                                               int ignore_done) {
+                                               int ignore_done, int form_xplus) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (!ignore_done && nd.done) return;
@@ -361,26 +224,34 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
   if (n <= 0) return;
   const int n2 = n + (n & 1), h = n2 >> 1;
   const int tid = threadIdx.x;
-  double* rc = smem;           // h
-  double* rs = rc + 512;       // h
-  double* rt = rs + 512;       // h
-  double* red = rt + 512;      // 64
-  double* tA = red + 64;       // 16*65
-  double* tB = tA + 16 * 65;   // 16*65
-  double* A = IN_LDS ? (tB + 16 * 65) : nd.S2;
-  double* __restrict__ V = nd.V;
+  double* rc = smem;
+  double* rs = rc + h;
+  double* rt = rs + h;
+  int* ptab = (int*)(rt + h);  // ptab[k] = p, ptab[h + k] = q
+  double* red = rt + 2 * h;
+  double* tA = red + 64;
+  double* tB = tA + 16 * 65;
+  double* lds_tail = tB + 16 * 65;
+  const size_t npk = (size_t)n2 * (n2 + 1) / 2;
+  double* A = A_LDS ? lds_tail : nd.S2;
+  double* V = V_LDS ? (lds_tail + (A_LDS ? npk : 0)) : nd.V;
   const double* __restrict__ Wm = nd.W;
   const bool warm = !values_only && nd.eig_warm;
+  const float inv_n = 1.0f / (float)n;
 
-  // ---- form A
+  // ---- form A (and V)
   if (warm) {
-    wg_gemm<false, false>(Wm, n, V, n, nullptr, n, n, n, DenseOut{nd.S1, n}, tA, tB);
+    wg_gemm<false, false>(Wm, n, nd.V, n, nullptr, n, n, n, DenseOut{nd.S1, n}, tA, tB);
     __syncthreads();
-    wg_gemm<true, false>(V, n, nd.S1, n, nullptr, n, n, n, PackedOut{A, n2}, tA, tB);
+    wg_gemm<true, false>(nd.V, n, nd.S1, n, nullptr, n, n, n, PackedOut{A, n2}, tA, tB);
+    if (V_LDS)
+      for (int f = tid; f < n * n; f += WG) V[f] = nd.V[f];
   } else {
     for (int f = tid; f < n * n; f += WG) {
-      const int i = f % n, j = f / n;
-      if (i >= j) A[pk(i, j, n2)] = Wm[i + (size_t)j * n];
+      int j = (int)((f + 0.5f) * inv_n);
+      int i = f - j * n;
+      if (i < 0) { --j; i += n; } else if (i >= n) { ++j; i -= n; }
+      if (i >= j) A[pk(i, j, n2)] = Wm[f];
       if (!values_only) V[f] = (i == j) ? 1.0 : 0.0;
     }
   }
@@ -388,20 +259,20 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
     for (int j = tid; j < n2; j += WG) A[pk(n, j, n2)] = 0.0;
   __syncthreads();
 
-  // ---- norms
+  // ---- norms over the packed triangle
   double nv[2] = {0.0, 0.0};  // off^2, total^2
-  for (int f = tid; f < n * n; f += WG) {
-    const int i = f % n, j = f / n;
-    if (i < j) continue;
-    const double a = A[pk(i, j, n2)];
-    const double a2 = (i == j) ? a * a : 2.0 * a * a;
-    nv[1] += a2;
-    if (i != j) nv[0] += a2;
+  for (int j = 0; j < n; ++j) {
+    const int base = pk(j, j, n2);
+    for (int i = j + tid; i < n; i += WG) {
+      const double a = A[base + (i - j)];
+      const double a2 = (i == j) ? a * a : 2.0 * a * a;
+      nv[1] += a2;
+      if (i != j) nv[0] += a2;
+    }
   }
   block_sums<2>(nv, red);
   double off2 = nv[0];
-  const double fro2 = nv[1];
-  const double thr = 1e-28 * fro2;  // relative off-norm 1e-14
+  const double thr = 1e-20 * nv[1];
   const int nblk = h * (h + 1) / 2;
   int sweep = 0;
   for (; sweep < max_sweeps && off2 > thr; ++sweep) {
@@ -409,6 +280,8 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
       for (int k = tid; k < h; k += WG) {
         int p, q;
         rr_pair(r, k, n2, p, q);
+        ptab[k] = p;
+        ptab[h + k] = q;
         double c = 1.0, s = 0.0, t = 0.0;
         if (p < n && q < n) {
           const double apq = A[pk(p, q, n2)];
@@ -426,10 +299,11 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
       }
       __syncthreads();
       for (int f = tid; f < nblk; f += WG) {
-        int k, l;
-        tri_decode(f, k, l);
-        int p, q, p2, q2;
-        rr_pair(r, k, n2, p, q);
+        int k = (int)((sqrtf(8.0f * (float)f + 1.0f) - 1.0f) * 0.5f);
+        if ((k + 1) * (k + 2) / 2 <= f) ++k;
+        if (k * (k + 1) / 2 > f) --k;
+        const int l = f - k * (k + 1) / 2;
+        const int p = ptab[k], q = ptab[h + k];
         if (k == l) {
           const int ipq = pk(p, q, n2);
           const double apq = A[ipq], t = rt[k];
@@ -438,7 +312,7 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
           A[iqq] = A[iqq] + t * apq;
           A[ipq] = 0.0;
         } else {
-          rr_pair(r, l, n2, p2, q2);
+          const int p2 = ptab[l], q2 = ptab[h + l];
           const double c1 = rc[k], s1 = rs[k], c2 = rc[l], s2 = rs[l];
           const int i00 = pk(p, p2, n2), i01 = pk(p, q2, n2), i10 = pk(q, p2, n2),
                     i11 = pk(q, q2, n2);
@@ -453,9 +327,10 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
       }
       if (!values_only)
         for (int f = tid; f < n * h; f += WG) {
-          const int i = f % n, l = f / n;
-          int p, q;
-          rr_pair(r, l, n2, p, q);
+          int l = (int)((f + 0.5f) * inv_n);
+          int i = f - l * n;
+          if (i < 0) { --l; i += n; } else if (i >= n) { ++l; i -= n; }
+          const int p = ptab[l], q = ptab[h + l];
           if (p >= n || q >= n) continue;
           const double c = rc[l], s = rs[l];
           const double vp = V[i + (size_t)p * n], vq = V[i + (size_t)q * n];
@@ -465,11 +340,12 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
       __syncthreads();
     }
     double ov[1] = {0.0};
-    for (int f = tid; f < n * n; f += WG) {
-      const int i = f % n, j = f / n;
-      if (i <= j) continue;
-      const double a = A[pk(i, j, n2)];
-      ov[0] += 2.0 * a * a;
+    for (int j = 0; j < n; ++j) {
+      const int base = pk(j, j, n2);
+      for (int i = j + 1 + tid; i < n; i += WG) {
+        const double a = A[base + (i - j)];
+        ov[0] += 2.0 * a * a;
+      }
     }
     block_sums<1>(ov, red);
     off2 = ov[0];
@@ -481,33 +357,79 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
     nd.sweeps_total += sweep;
     if (!values_only) nd.eig_warm = 1;
   }
+  if (!values_only && form_xplus) {
+    // lambda_+ into the rotation arrays (3h >= n doubles, free after the sweeps)
+    double* dpos = rc;
+    __syncthreads();
+    double mv[1] = {0.0};
+    for (int j = tid; j < n; j += WG) {
+      const double lam = A[pk(j, j, n2)];
+      const double d = lam > 1e-8 ? lam : 0.0;
+      dpos[j] = d;
+      nd.evp[j] = d;
+      mv[0] += d > 0.0 ? 1.0 : 0.0;
+    }
+    block_sums<1>(mv, red);
+    if (tid == 0) nd.M = (int)mv[0];
+    if (V_LDS) {
+      // register-blocked X_+ (4x4 per thread) straight from LDS V
+      const int nb4 = (n + 3) >> 2;
+      const int nbl = nb4 * (nb4 + 1) / 2;
+      for (int f = tid; f < nbl; f += WG) {
+        int bi = (int)((sqrtf(8.0f * (float)f + 1.0f) - 1.0f) * 0.5f);
+        if ((bi + 1) * (bi + 2) / 2 <= f) ++bi;
+        if (bi * (bi + 1) / 2 > f) --bi;
+        const int bj = f - bi * (bi + 1) / 2;
+        const int i0 = bi * 4, j0 = bj * 4;
+        double acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int kk = 0; kk < n; ++kk) {
+          const double dk = dpos[kk];
+          if (dk == 0.0) continue;
+          const double* col = V + (size_t)kk * n;
+          double vi[4], vj[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            vi[a] = (i0 + a < n) ? col[i0 + a] * dk : 0.0;
+            vj[a] = (j0 + a < n) ? col[j0 + a] : 0.0;
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(vi[a], vj[b], acc[a][b]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int i = i0 + a, j = j0 + b;
+            if (i < n && j < n && i >= j) {
+              nd.S1[i + (size_t)j * n] = acc[a][b];
+              nd.S1[j + (size_t)i * n] = acc[a][b];
+            }
+          }
+      }
+    } else {
+      wg_gemm<false, true>(V, n, V, n, dpos, n, n, n, DenseOut{nd.S1, n}, tA, tB);
+    }
+  }
+  if (V_LDS && !values_only) {
+    __syncthreads();
+    for (int f = tid; f < n * n; f += WG) nd.V[f] = V[f];
+  }
 }
 
-// X_+ = V diag(lambda_+) V^T with lambda_+ = lambda for lambda > 1e-8, else 0
-// (the dsyevr range (1e-8, VU] of operators.c:536-579; VU >= lambda_max);
-// then X = sigma X_+, Q = X_+ - W and the residuals, sigma update and stop
-// test of admm.c:580-702.
+// After k_jacobi formed X_+ in S1: X = sigma X_+, Q = X_+ - W and the
+// residuals, sigma update and stop test of admm.c:580-702.
 __global__ __launch_bounds__(WG) void k_admm_post(NodeView* nodes) {
   __shared__ double red[4 * 16];
-  __shared__ double tA[16 * 65], tB[16 * 65];
-  __shared__ double dpos[1024];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
   const int n = nd.n, m = nd.m, tid = threadIdx.x;
   const double sg = nd.sigma;
-  // positive eigenvalues (n <= 1024)
-  int mloc = 0;
-  for (int j = tid; j < n; j += WG) {
-    const double lam = nd.ev[j];
-    const double d = lam > 1e-8 ? lam : 0.0;
-    dpos[j] = d;
-    nd.evp[j] = d;
-    mloc += d > 0.0;
-  }
-  double mv[1] = {(double)mloc};
-  block_sums<1>(mv, red);
-  wg_gemm<false, true>(nd.V, n, nd.V, n, dpos, n, n, n, DenseOut{nd.S1, n}, tA, tB);
-  __syncthreads();
   double v3[3] = {0, 0, 0};  // |X+ - Xold/sigma|_F^2, |diag X - 1|^2, <L,X>
   for (int f = tid; f < n * n; f += WG) {
     const int i = f % n, j = f / n;
@@ -550,7 +472,6 @@ __global__ __launch_bounds__(WG) void k_admm_post(NodeView* nodes) {
   for (int i = tid; i < n; i += WG) v4[3] += nd.y[i];
   block_sums<4>(v4, red);
   if (tid == 0) {
-    nd.M = (int)mv[0];
     nd.sigma_last = sg;
     const double err_p = (sqrt(v3[1]) + sqrt(v4[0])) / (1.0 + sqrt((double)n));
     const double err_d = (sqrt(v4[1]) + sqrt(v3[0])) / (1.0 + nd.normL);
@@ -645,35 +566,16 @@ __global__ __launch_bounds__(WGS) void k_op_Bt(const NodeView* nodes, const doub
   }
 }
 
-// Projection only (project_SDP, operators.c:521-619): after k_jacobi on W,
-// X = X_+, Q = X_+ - W (no sigma scaling).
-__global__ __launch_bounds__(WG) void k_project_out(NodeView* nodes) {
-  __shared__ double tA[16 * 65], tB[16 * 65];
-  __shared__ double dpos[1024];
-  __shared__ double red[64];
+// Projection only (project_SDP, operators.c:521-619): after k_jacobi formed
+// X_+ in S1, X = X_+ and Q = X_+ - W (no sigma scaling).
+__global__ __launch_bounds__(WGS) void k_project_out(NodeView* nodes) {
   NodeView& nd = nodes[blockIdx.x];
-  const int n = nd.n, tid = threadIdx.x;
-  int mloc = 0;
-  for (int j = tid; j < n; j += WG) {
-    const double lam = nd.ev[j];
-    dpos[j] = lam > 1e-8 ? lam : 0.0;
-    mloc += lam > 1e-8;
+  const int n = nd.n;
+  for (int f = threadIdx.x; f < n * n; f += WGS) {
+    const double xp = nd.S1[f];
+    nd.X[f] = xp;
+    nd.Q[f] = xp - nd.W[f];
   }
-  double mv[1] = {(double)mloc};
-  block_sums<1>(mv, red);
-  wg_gemm<false, true>(nd.V, n, nd.V, n, dpos, n, n, n, DenseOut{nd.S1, n}, tA, tB);
-  __syncthreads();
-  for (int f = tid; f < n * n; f += WG) {
-    const int i = f % n, j = f / n;
-    if (i < j) continue;
-    const double xp = nd.S1[i + j * n];
-    nd.X[i + j * n] = xp;
-    nd.X[j + i * n] = xp;
-    const double qv = xp - nd.W[i + j * n];
-    nd.Q[i + j * n] = qv;
-    nd.Q[j + i * n] = qv;
-  }
-  if (tid == 0) nd.M = (int)mv[0];
 }
 
 // ------------------------------------------------------------ launchers
@@ -700,21 +602,26 @@ void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipS
   k_build_w<<<grid, WGS, 0, st>>>(nodes, use_s);
 }
 
-static size_t jacobi_lds_fixed() { return (3 * 512 + 64 + 2 * 16 * 65) * sizeof(double); }
-
-size_t jacobi_lds_bytes(int nmax) {
-  const int n2 = nmax + (nmax & 1);
-  return jacobi_lds_fixed() + (size_t)n2 * (n2 + 1) / 2 * sizeof(double);
+size_t jacobi_lds_bytes(int nmax, bool with_v) {
+  const int n2 = nmax + (nmax & 1), h = n2 / 2;
+  size_t d = jac_fixed_doubles(h) + (size_t)n2 * (n2 + 1) / 2;
+  if (with_v) d += (size_t)nmax * nmax;
+  return d * sizeof(double);
 }
 
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
-                   int ignore_done, size_t lds_limit, hipStream_t st) {
-  const size_t need = jacobi_lds_bytes(nmax);
-  if (need <= lds_limit) {
-    k_jacobi<true><<<nb, WG, need, st>>>(nodes, values_only, max_sweeps, ignore_done);
+                   int ignore_done, size_t lds_limit, hipStream_t st, int form_xplus) {
+  const int h = (nmax + (nmax & 1)) / 2;
+  const size_t fixed = jac_fixed_doubles(h) * sizeof(double);
+  if (!values_only && jacobi_lds_bytes(nmax, true) <= lds_limit) {
+    k_jacobi<true, true><<<nb, WG, jacobi_lds_bytes(nmax, true), st>>>(
+        nodes, values_only, max_sweeps, ignore_done, form_xplus);
+  } else if (jacobi_lds_bytes(nmax, false) <= lds_limit) {
+    k_jacobi<true, false><<<nb, WG, jacobi_lds_bytes(nmax, false), st>>>(
+        nodes, values_only, max_sweeps, ignore_done, form_xplus);
   } else {
-    k_jacobi<false><<<nb, WG, jacobi_lds_fixed(), st>>>(nodes, values_only, max_sweeps,
-                                                      ignore_done);
+    k_jacobi<false, false><<<nb, WG, fixed, st>>>(nodes, values_only, max_sweeps, ignore_done,
+                                                  form_xplus);
   }
 }
 
@@ -741,7 +648,7 @@ void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout
 
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   (void)nmax;
-  k_project_out<<<nb, WG, 0, st>>>(nodes);
+  k_project_out<<<nb, WGS, 0, st>>>(nodes);
 }
 
 }  // namespace madam

@@ -1,0 +1,158 @@
+// madam_devutil.h -- device helpers shared by the HIP kernels: workgroup
+// reductions, packed-triangle indexing, round-robin pairing and an LDS-tiled
+// FP64 GEMM for one workgroup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace madam {
+
+constexpr int WG = 1024;   // per-node kernels (16 waves)
+constexpr int WGS = 256;   // streaming kernels
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum of NV values over the workgroup; every thread receives the totals.
+// `red` must hold NV * (WG/64) doubles.  Deterministic order.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[q * 16 + w] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double r = 0.0;
+    for (int i = 0; i < nw; ++i) r += red[q * 16 + i];
+    v[q] = r;
+  }
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int i = 1; i < nw; ++i) r = fmax(r, red[i]);
+  return r;
+}
+
+__device__ __forceinline__ int ent_idx(int e) { return e >= 0 ? e : ~e; }
+
+// Packed lower-triangular index (column-major) for an order-n2 matrix.
+__device__ __forceinline__ int pk(int i, int j, int n2) {
+  if (i < j) {
+    int t = i;
+    i = j;
+    j = t;
+  }
+  return i + j * (n2 - 1) - ((j * (j - 1)) >> 1);
+}
+
+// Round-robin (circle method) pairing: round r, pair k, n2 even.
+__device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
+  const int N1 = n2 - 1;
+  if (k == 0) {
+    p = N1;
+    q = r;
+  } else {
+    p = (r + k) % N1;
+    q = (r - k + N1) % N1;
+  }
+}
+
+// f -> (k, l) with k >= l, f = k(k+1)/2 + l.
+__device__ __forceinline__ void tri_decode(int f, int& k, int& l) {
+  int kk = (int)((sqrt(8.0 * f + 1.0) - 1.0) * 0.5);
+  while ((kk + 1) * (kk + 2) / 2 <= f) ++kk;
+  while (kk * (kk + 1) / 2 > f) --kk;
+  k = kk;
+  l = f - kk * (kk + 1) / 2;
+}
+
+// -------------------------------------------------------- tiled FP64 GEMM
+// C(MxN) = op(A) * diag(d) * op(B) over K, one workgroup of WG threads.
+// op(A)[i,k] = TA ? A[k + i*lda] : A[i + k*lda]; op(B)[k,j] = TB ? B[j + k*ldb]
+// : B[k + j*ldb]; d may be null.  `out(i, j, v)` stores a result.  Tiles of
+// 64x64 outputs, 16-deep k slabs staged in LDS (tA, tB: 16*65 doubles each).
+template <bool TA, bool TB, class Out>
+__device__ void wg_gemm(const double* __restrict__ A, int lda, const double* __restrict__ B,
+                        int ldb, const double* __restrict__ d, int M, int N, int K, Out out,
+                        double* tA, double* tB) {
+  const int tid = threadIdx.x;
+  const int ti = tid & 31, tj = tid >> 5;  // 32 x 32 threads, each 2x2 outputs
+  for (int i0 = 0; i0 < M; i0 += 64)
+    for (int j0 = 0; j0 < N; j0 += 64) {
+      double acc00 = 0, acc01 = 0, acc10 = 0, acc11 = 0;
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        {  // A slab: 64 (i) x 16 (k)
+          int i, kk;
+          if (TA) {
+            kk = tid & 15;
+            i = tid >> 4;
+          } else {
+            i = tid & 63;
+            kk = tid >> 6;
+          }
+          const int gi = i0 + i, gk = k0 + kk;
+          double v = 0.0;
+          if (gi < M && gk < K) {
+            v = TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda];
+            if (d) v *= d[gk];
+          }
+          tA[kk * 65 + i] = v;
+        }
+        {  // B slab: 16 (k) x 64 (j)
+          int j, kk;
+          if (TB) {
+            j = tid & 63;
+            kk = tid >> 6;
+          } else {
+            kk = tid & 15;
+            j = tid >> 4;
+          }
+          const int gj = j0 + j, gk = k0 + kk;
+          double v = 0.0;
+          if (gj < N && gk < K) v = TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb];
+          tB[kk * 65 + j] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+          const double a0 = tA[kk * 65 + ti], a1 = tA[kk * 65 + ti + 32];
+          const double b0 = tB[kk * 65 + tj], b1 = tB[kk * 65 + tj + 32];
+          acc00 = fma(a0, b0, acc00);
+          acc01 = fma(a0, b1, acc01);
+          acc10 = fma(a1, b0, acc10);
+          acc11 = fma(a1, b1, acc11);
+        }
+        __syncthreads();
+      }
+      const int r0 = i0 + ti, r1 = i0 + ti + 32, c0 = j0 + tj, c1 = j0 + tj + 32;
+      if (r0 < M && c0 < N) out(r0, c0, acc00);
+      if (r0 < M && c1 < N) out(r0, c1, acc01);
+      if (r1 < M && c0 < N) out(r1, c0, acc10);
+      if (r1 < M && c1 < N) out(r1, c1, acc11);
+    }
+}
+
+
+}  // namespace madam

@@ -1,0 +1,415 @@
+// madam_engine.h -- host engine internals of libmadam_bound.so: device
+// buffers, node slots, the per-phase timer and the cut-structure build.
+// Shared by the C ABI (madam_host.hip) and the bounding driver
+// (madam_bab.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/madam_bound.h"
+#include "madam_gw.h"
+#include "madam_internal.h"
+#include "madam_sep.h"
+
+namespace mh {
+
+using madam::NodeView;
+
+
+extern char g_err[512];
+
+inline void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_err("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));   \
+      return MADAM_E_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+// ---------------------------------------------------------------- cut rows
+
+// Host image of the cut structure of one node (see NodeView).
+struct CutSet {
+  int m = 0, nact = 0;
+  std::vector<int> cptr, cent, cact, aptr, aent, apair, p2a;
+  std::vector<double> cscale, y0;
+};
+
+// Vertex tuple and sign vector (operators.c:85-148): triangle types 2/3/4
+// negate vertex k/j/i; pentagonal/heptagonal type t negates the first t-1.
+inline int tri_expand(const madam_tri& c, int* v, int* b) {
+  v[0] = c.i;
+  v[1] = c.j;
+  v[2] = c.k;
+  b[0] = b[1] = b[2] = 1;
+  if (c.type == 2) b[2] = -1;
+  else if (c.type == 3) b[1] = -1;
+  else if (c.type == 4) b[0] = -1;
+  return 3;
+}
+inline int kgon_expand(int k, int type, const int* perm, int* v, int* b) {
+  for (int a = 0; a < k; ++a) {
+    v[a] = perm[a];
+    b[a] = (a < type - 1) ? -1 : 1;
+  }
+  return k;
+}
+
+inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const madam_pent* pent,
+                 int nhept, const madam_hept* hept, CutSet& cs) {
+  const int m = ntri + npent + nhept;
+  cs.m = m;
+  cs.cptr.assign(m + 1, 0);
+  cs.cent.clear();
+  cs.cscale.resize(m);
+  cs.y0.resize(m);
+  cs.p2a.assign((size_t)n * n, -1);
+  cs.apair.clear();
+  std::vector<int> lo_list;
+  lo_list.reserve((size_t)m * 21);
+  int v[7], b[7];
+  for (int c = 0; c < m; ++c) {
+    int k;
+    double sc, y0;
+    if (c < ntri) {
+      k = tri_expand(tri[c], v, b);
+      sc = 1.0;
+      y0 = tri[c].y;
+    } else if (c < ntri + npent) {
+      const madam_pent& P = pent[c - ntri];
+      k = kgon_expand(5, P.type, P.perm, v, b);
+      sc = 0.5;
+      y0 = P.y;
+    } else {
+      const madam_hept& H = hept[c - ntri - npent];
+      k = kgon_expand(7, H.type, H.perm, v, b);
+      sc = 1.0 / 3.0;
+      y0 = H.y;
+    }
+    for (int a = 0; a < k; ++a)
+      if (v[a] < 0 || v[a] >= n) {
+        set_err("cut %d has vertex %d outside 0..%d", c, v[a], n - 1);
+        return MADAM_E_ARG;
+      }
+    cs.cscale[c] = sc;
+    cs.y0[c] = y0;
+    for (int a = 0; a < k; ++a)
+      for (int q = a + 1; q < k; ++q) {
+        const int hi = std::max(v[a], v[q]), lw = std::min(v[a], v[q]);
+        if (hi == lw) {
+          set_err("cut %d repeats vertex %d", c, hi);
+          return MADAM_E_ARG;
+        }
+        const int lo = hi + n * lw;
+        const int sgn = -(b[a] * b[q]);
+        cs.cent.push_back(sgn > 0 ? lo : ~lo);
+        lo_list.push_back(lo);
+        if (cs.p2a[lo] < 0) {
+          cs.p2a[lo] = (int)cs.apair.size();
+          cs.apair.push_back(lo);
+        }
+      }
+    cs.cptr[c + 1] = (int)cs.cent.size();
+  }
+  const int nact = (int)cs.apair.size();
+  cs.nact = nact;
+  cs.cact.resize(cs.cent.size());
+  cs.aptr.assign(nact + 1, 0);
+  for (size_t e = 0; e < cs.cent.size(); ++e) {
+    const int a = cs.p2a[lo_list[e]];
+    cs.cact[e] = a;
+    cs.aptr[a + 1]++;
+  }
+  for (int a = 0; a < nact; ++a) cs.aptr[a + 1] += cs.aptr[a];
+  cs.aent.assign(cs.cent.size(), 0);
+  std::vector<int> fill(cs.aptr.begin(), cs.aptr.end() - 1);
+  for (int c = 0; c < m; ++c)
+    for (int e = cs.cptr[c]; e < cs.cptr[c + 1]; ++e) {
+      const int a = cs.cact[e];
+      cs.aent[fill[a]++] = cs.cent[e] >= 0 ? c : ~c;
+    }
+  return MADAM_OK;
+}
+
+// ---------------------------------------------------------------- engine
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n) {
+    if (n <= cap) return MADAM_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(n, 64);
+    HIP_TRY(hipMalloc(&p, want * sizeof(T)));
+    cap = want;
+    return MADAM_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Slot {
+  DBuf<double> L, X, Q, W, V, S1, S2, ev, evp, y;
+  DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
+  DBuf<double> cscale, t, s, u, rhs, r, p, q, w;
+  DBuf<double> ip1, ip2, ip3, ipv, ipP;  // IPM scratch
+  DBuf<double> gwval, gwbest;            // GW per-hyperplane values, best value
+  DBuf<signed char> gwz, gwbz;           // GW cuts, best cut
+  int n_ipm = 0;
+
+  int size_ipm(int n) {
+    const size_t nn = (size_t)n * n;
+    int rc;
+    if ((rc = ip1.ensure(nn)) || (rc = ip2.ensure(nn)) || (rc = ip3.ensure(nn)) ||
+        (rc = ipv.ensure(4 * (size_t)n)) || (rc = ipP.ensure(nn)))
+      return rc;
+    return MADAM_OK;
+  }
+  int size_gw(int n, int num) {
+    int rc;
+    if ((rc = gwval.ensure(num)) || (rc = gwbest.ensure(1)) || (rc = gwz.ensure((size_t)num * n)) ||
+        (rc = gwbz.ensure(n)))
+      return rc;
+    return MADAM_OK;
+  }
+
+  int size_for(int n, const CutSet& cs) {
+    const size_t nn = (size_t)n * n;
+    int rc = 0;
+    for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2})
+      if ((rc = b->ensure(nn))) return rc;
+    if ((rc = ev.ensure(n)) || (rc = evp.ensure(n)) || (rc = y.ensure(n))) return rc;
+    const size_t m = std::max(cs.m, 1), ne = std::max<size_t>(cs.cent.size(), 1),
+                 na = std::max(cs.nact, 1);
+    if ((rc = cptr.ensure(m + 1)) || (rc = cent.ensure(ne)) || (rc = cact.ensure(ne)) ||
+        (rc = aptr.ensure(na + 1)) || (rc = aent.ensure(ne)) || (rc = apair.ensure(na)) ||
+        (rc = p2a.ensure(nn)))
+      return rc;
+    for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q})
+      if ((rc = b->ensure(m))) return rc;
+    return w.ensure(na);
+  }
+  void release() {
+    for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2, &ev, &evp, &y, &cscale, &t, &s, &u, &rhs,
+                            &r, &p, &q, &w})
+      b->release();
+    for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a}) b->release();
+    for (DBuf<double>* b : {&ip1, &ip2, &ip3, &ipv, &ipP, &gwval, &gwbest}) b->release();
+    gwz.release();
+    gwbz.release();
+  }
+};
+
+// Optional per-phase GPU timing with HIP events on the engine's stream.
+enum Phase { PH_PREP, PH_CG, PH_W, PH_EIG, PH_POST, PH_SAFE, PH_IPM, PH_TRISEP, PH_SA, PH_GW,
+             PH_COUNT };
+
+struct PhaseTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double ms[PH_COUNT] = {0};
+  long count[PH_COUNT] = {0};
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // resolve recorded pairs; the stream must be synchronised by the caller
+  void flush() {
+    for (auto& pr : pending) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, pr.second.first, pr.second.second);
+      ms[pr.first] += t;
+      count[pr.first] += 1;
+      pool.push_back(pr.second.first);
+      pool.push_back(pr.second.second);
+    }
+    pending.clear();
+  }
+};
+
+// Per-slot separation buffers.
+struct SepSlot {
+  DBuf<uint32_t> ctl, hist, okey, bkey, fkey;
+  DBuf<unsigned long long> maxbits;
+  DBuf<double> bval, fval;
+  int ensure(int K) {
+    int rc;
+    if ((rc = ctl.ensure(madam::CTL_N)) || (rc = hist.ensure(madam::SEP_NBIN)) ||
+        (rc = okey.ensure(K)) || (rc = bkey.ensure(madam::SEP_BCAP)) || (rc = fkey.ensure(K)) ||
+        (rc = maxbits.ensure(1)) || (rc = bval.ensure(madam::SEP_BCAP)) || (rc = fval.ensure(K)))
+      return rc;
+    return MADAM_OK;
+  }
+  void release() {
+    ctl.release(); hist.release(); okey.release(); bkey.release(); fkey.release();
+    maxbits.release(); bval.release(); fval.release();
+  }
+};
+
+struct Engine {
+  std::vector<SepSlot> sep;
+  DBuf<madam::SepView> sepviews;
+  DBuf<madam::SaJob> sajobs;
+  DBuf<madam::GwNode> gwnodes;
+  DBuf<madam::SaOut> saout;
+  DBuf<double> sumx;
+  DBuf<const double*> xptrs;
+  DBuf<int> nvals;
+  PhaseTimer timer;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  size_t lds_limit = 0;
+  std::vector<Slot> slots;
+  DBuf<NodeView> views;
+  DBuf<double> tmpd;
+  std::vector<NodeView> hviews;
+};
+
+extern Engine* g_eng;
+
+// Run `call` (kernel launches on stream st) bracketed by events when timing.
+#define TIMED(phase, st, call)                                          \
+  do {                                                                  \
+    if (g_eng->timer.on) {                                              \
+      hipEvent_t a_ = g_eng->timer.get(), b_ = g_eng->timer.get();      \
+      (void)hipEventRecord(a_, st);                                     \
+      call;                                                             \
+      (void)hipEventRecord(b_, st);                                     \
+      g_eng->timer.pending.push_back({phase, {a_, b_}});                \
+    } else {                                                            \
+      call;                                                             \
+    }                                                                   \
+  } while (0)
+
+inline int ensure_open() {
+  if (g_eng) return MADAM_OK;
+  return madam_open(0);
+}
+
+inline madam_opts default_opts() {
+  madam_opts o;
+  o.cg_rtol = 1e-12;
+  o.cg_maxit = 1000;
+  o.jacobi_sweeps = 40;
+  o.check_every = 16;
+  return o;
+}
+
+template <class T>
+inline int up(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n == 0) return MADAM_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return MADAM_OK;
+}
+template <class T>
+inline int down(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n == 0) return MADAM_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  return MADAM_OK;
+}
+
+// Fill view v from slot sl (device pointers) for a node of order n.
+inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
+  memset(&v, 0, sizeof v);
+  v.n = n;
+  v.m = cs.m;
+  v.nact = cs.nact;
+  v.L = sl.L.p;
+  v.X = sl.X.p;
+  v.Q = sl.Q.p;
+  v.W = sl.W.p;
+  v.V = sl.V.p;
+  v.S1 = sl.S1.p;
+  v.S2 = sl.S2.p;
+  v.ev = sl.ev.p;
+  v.evp = sl.evp.p;
+  v.y = sl.y.p;
+  v.cptr = sl.cptr.p;
+  v.cent = sl.cent.p;
+  v.cact = sl.cact.p;
+  v.cscale = sl.cscale.p;
+  v.aptr = sl.aptr.p;
+  v.aent = sl.aent.p;
+  v.apair = sl.apair.p;
+  v.p2a = sl.p2a.p;
+  v.t = sl.t.p;
+  v.s = sl.s.p;
+  v.u = sl.u.p;
+  v.rhs = sl.rhs.p;
+  v.cg_r = sl.r.p;
+  v.cg_p = sl.p.p;
+  v.cg_q = sl.q.p;
+  v.cg_w = sl.w.p;
+  v.ip1 = sl.ip1.p;
+  v.ip2 = sl.ip2.p;
+  v.ip3 = sl.ip3.p;
+  v.ipv = sl.ipv.p;
+  v.ipP = sl.ipP.p;
+}
+
+inline int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {
+  int rc;
+  if ((rc = up(sl.cptr.p, cs.cptr.data(), cs.cptr.size(), s)) ||
+      (rc = up(sl.cent.p, cs.cent.data(), cs.cent.size(), s)) ||
+      (rc = up(sl.cact.p, cs.cact.data(), cs.cact.size(), s)) ||
+      (rc = up(sl.aptr.p, cs.aptr.data(), cs.aptr.size(), s)) ||
+      (rc = up(sl.aent.p, cs.aent.data(), cs.aent.size(), s)) ||
+      (rc = up(sl.apair.p, cs.apair.data(), cs.apair.size(), s)) ||
+      (rc = up(sl.p2a.p, cs.p2a.data(), cs.p2a.size(), s)) ||
+      (rc = up(sl.cscale.p, cs.cscale.data(), cs.cscale.size(), s)))
+    return rc;
+  return MADAM_OK;
+}
+
+inline int ensure_slots(int k) {
+  if ((int)g_eng->slots.size() < k) g_eng->slots.resize(k);
+  g_eng->hviews.resize(std::max<size_t>(g_eng->hviews.size(), k));
+  return g_eng->views.ensure(k);
+}
+
+inline int sync_views_down(int k) {
+  int rc = down(g_eng->hviews.data(), g_eng->views.p, k, g_eng->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g_eng->stream));
+  return MADAM_OK;
+}
+
+
+// The ADMM loop of admm.c:520-704 plus the safe bound (admm.c:724-763) for
+// the k nodes whose views are in hv (device buffers already filled: L, X,
+// Q, y, cut structure, s = t = cut duals, u = 0).  hv is updated in place.
+int admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
+
+// Triangle separation on the given slots (X current on the device).
+int tri_separate_slots(const std::vector<int>& slots, const int* K, double thresh,
+                       std::vector<std::vector<madam_tri>>& lists, std::vector<double>& maxval);
+
+}  // namespace mh

@@ -6,7 +6,7 @@
 // admm.c:520-704 for all of them with the same kernel launches, checking the
 // per-node stop flags on the host every `check_every` iterations; nodes that
 // stopped skip the remaining launches inside the kernels.
-#include "madam_internal.h"
+#include "madam_engine.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -24,339 +24,57 @@ static_assert(sizeof(madam_tri) == 32, "Triangle_Inequality layout");
 static_assert(sizeof(madam_pent) == 40, "Pentagonal_Inequality layout");
 static_assert(sizeof(madam_hept) == 48, "Heptagonal_Inequality layout");
 
-namespace madam {
-size_t jacobi_lds_bytes(int nmax);
-}
 
-using madam::NodeView;
-
-namespace {
-
+namespace mh {
 char g_err[512] = "";
-
-void set_err(const char* fmt, ...) {
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(g_err, sizeof g_err, fmt, ap);
-  va_end(ap);
-}
-
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess) {                                                        \
-      set_err("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));   \
-      return MADAM_E_HIP;                                                          \
-    }                                                                              \
-  } while (0)
-
-// ---------------------------------------------------------------- cut rows
-
-// Host image of the cut structure of one node (see NodeView).
-struct CutSet {
-  int m = 0, nact = 0;
-  std::vector<int> cptr, cent, cact, aptr, aent, apair, p2a;
-  std::vector<double> cscale, y0;
-};
-
-// Vertex tuple and sign vector (operators.c:85-148): triangle types 2/3/4
-// negate vertex k/j/i; pentagonal/heptagonal type t negates the first t-1.
-int tri_expand(const madam_tri& c, int* v, int* b) {
-  v[0] = c.i;
-  v[1] = c.j;
-  v[2] = c.k;
-  b[0] = b[1] = b[2] = 1;
-  if (c.type == 2) b[2] = -1;
-  else if (c.type == 3) b[1] = -1;
-  else if (c.type == 4) b[0] = -1;
-  return 3;
-}
-int kgon_expand(int k, int type, const int* perm, int* v, int* b) {
-  for (int a = 0; a < k; ++a) {
-    v[a] = perm[a];
-    b[a] = (a < type - 1) ? -1 : 1;
-  }
-  return k;
-}
-
-int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const madam_pent* pent,
-                 int nhept, const madam_hept* hept, CutSet& cs) {
-  const int m = ntri + npent + nhept;
-  cs.m = m;
-  cs.cptr.assign(m + 1, 0);
-  cs.cent.clear();
-  cs.cscale.resize(m);
-  cs.y0.resize(m);
-  cs.p2a.assign((size_t)n * n, -1);
-  cs.apair.clear();
-  std::vector<int> lo_list;
-  lo_list.reserve((size_t)m * 21);
-  int v[7], b[7];
-  for (int c = 0; c < m; ++c) {
-    int k;
-    double sc, y0;
-    if (c < ntri) {
-      k = tri_expand(tri[c], v, b);
-      sc = 1.0;
-      y0 = tri[c].y;
-    } else if (c < ntri + npent) {
-      const madam_pent& P = pent[c - ntri];
-      k = kgon_expand(5, P.type, P.perm, v, b);
-      sc = 0.5;
-      y0 = P.y;
-    } else {
-      const madam_hept& H = hept[c - ntri - npent];
-      k = kgon_expand(7, H.type, H.perm, v, b);
-      sc = 1.0 / 3.0;
-      y0 = H.y;
-    }
-    for (int a = 0; a < k; ++a)
-      if (v[a] < 0 || v[a] >= n) {
-        set_err("cut %d has vertex %d outside 0..%d", c, v[a], n - 1);
-        return MADAM_E_ARG;
-      }
-    cs.cscale[c] = sc;
-    cs.y0[c] = y0;
-    for (int a = 0; a < k; ++a)
-      for (int q = a + 1; q < k; ++q) {
-        const int hi = std::max(v[a], v[q]), lw = std::min(v[a], v[q]);
-        if (hi == lw) {
-          set_err("cut %d repeats vertex %d", c, hi);
-          return MADAM_E_ARG;
-        }
-        const int lo = hi + n * lw;
-        const int sgn = -(b[a] * b[q]);
-        cs.cent.push_back(sgn > 0 ? lo : ~lo);
-        lo_list.push_back(lo);
-        if (cs.p2a[lo] < 0) {
-          cs.p2a[lo] = (int)cs.apair.size();
-          cs.apair.push_back(lo);
-        }
-      }
-    cs.cptr[c + 1] = (int)cs.cent.size();
-  }
-  const int nact = (int)cs.apair.size();
-  cs.nact = nact;
-  cs.cact.resize(cs.cent.size());
-  cs.aptr.assign(nact + 1, 0);
-  for (size_t e = 0; e < cs.cent.size(); ++e) {
-    const int a = cs.p2a[lo_list[e]];
-    cs.cact[e] = a;
-    cs.aptr[a + 1]++;
-  }
-  for (int a = 0; a < nact; ++a) cs.aptr[a + 1] += cs.aptr[a];
-  cs.aent.assign(cs.cent.size(), 0);
-  std::vector<int> fill(cs.aptr.begin(), cs.aptr.end() - 1);
-  for (int c = 0; c < m; ++c)
-    for (int e = cs.cptr[c]; e < cs.cptr[c + 1]; ++e) {
-      const int a = cs.cact[e];
-      cs.aent[fill[a]++] = cs.cent[e] >= 0 ? c : ~c;
-    }
-  return MADAM_OK;
-}
-
-// ---------------------------------------------------------------- engine
-
-template <class T>
-struct DBuf {
-  T* p = nullptr;
-  size_t cap = 0;
-  int ensure(size_t n) {
-    if (n <= cap) return MADAM_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(n, 64);
-    HIP_TRY(hipMalloc(&p, want * sizeof(T)));
-    cap = want;
-    return MADAM_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
-
-struct Slot {
-  DBuf<double> L, X, Q, W, V, S1, S2, ev, evp, y;
-  DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
-  DBuf<double> cscale, t, s, u, rhs, r, p, q, w;
-
-  int size_for(int n, const CutSet& cs) {
-    const size_t nn = (size_t)n * n;
-    int rc = 0;
-    for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2})
-      if ((rc = b->ensure(nn))) return rc;
-    if ((rc = ev.ensure(n)) || (rc = evp.ensure(n)) || (rc = y.ensure(n))) return rc;
-    const size_t m = std::max(cs.m, 1), ne = std::max<size_t>(cs.cent.size(), 1),
-                 na = std::max(cs.nact, 1);
-    if ((rc = cptr.ensure(m + 1)) || (rc = cent.ensure(ne)) || (rc = cact.ensure(ne)) ||
-        (rc = aptr.ensure(na + 1)) || (rc = aent.ensure(ne)) || (rc = apair.ensure(na)) ||
-        (rc = p2a.ensure(nn)))
-      return rc;
-    for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q})
-      if ((rc = b->ensure(m))) return rc;
-    return w.ensure(na);
-  }
-  void release() {
-    for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2, &ev, &evp, &y, &cscale, &t, &s, &u, &rhs,
-                            &r, &p, &q, &w})
-      b->release();
-    for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a}) b->release();
-  }
-};
-
-// Optional per-phase GPU timing with HIP events on the engine's stream.
-enum Phase { PH_PREP, PH_CG, PH_W, PH_EIG, PH_POST, PH_SAFE, PH_IPM, PH_TRISEP, PH_SA, PH_GW,
-             PH_COUNT };
-
-struct PhaseTimer {
-  bool on = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-  double ms[PH_COUNT] = {0};
-  long count[PH_COUNT] = {0};
-  hipEvent_t get() {
-    if (pool.empty()) {
-      hipEvent_t e;
-      (void)hipEventCreate(&e);
-      return e;
-    }
-    hipEvent_t e = pool.back();
-    pool.pop_back();
-    return e;
-  }
-  // resolve recorded pairs; the stream must be synchronised by the caller
-  void flush() {
-    for (auto& pr : pending) {
-      float t = 0.f;
-      (void)hipEventElapsedTime(&t, pr.second.first, pr.second.second);
-      ms[pr.first] += t;
-      count[pr.first] += 1;
-      pool.push_back(pr.second.first);
-      pool.push_back(pr.second.second);
-    }
-    pending.clear();
-  }
-};
-
-struct Engine {
-  PhaseTimer timer;
-  int device = -1;
-  hipStream_t stream = nullptr;
-  size_t lds_limit = 0;
-  std::vector<Slot> slots;
-  DBuf<NodeView> views;
-  DBuf<double> tmpd;
-  std::vector<NodeView> hviews;
-};
-
 Engine* g_eng = nullptr;
+}  // namespace mh
+using namespace mh;
 
-// Run `call` (kernel launches on stream st) bracketed by events when timing.
-#define TIMED(phase, st, call)                                          \
-  do {                                                                  \
-    if (g_eng->timer.on) {                                              \
-      hipEvent_t a_ = g_eng->timer.get(), b_ = g_eng->timer.get();      \
-      (void)hipEventRecord(a_, st);                                     \
-      call;                                                             \
-      (void)hipEventRecord(b_, st);                                     \
-      g_eng->timer.pending.push_back({phase, {a_, b_}});                \
-    } else {                                                            \
-      call;                                                             \
-    }                                                                   \
-  } while (0)
-
-int ensure_open() {
-  if (g_eng) return MADAM_OK;
-  return madam_open(0);
-}
-
-madam_opts default_opts() {
-  madam_opts o;
-  o.cg_rtol = 1e-12;
-  o.cg_maxit = 1000;
-  o.jacobi_sweeps = 40;
-  o.check_every = 16;
-  return o;
-}
-
-template <class T>
-int up(T* dst, const T* src, size_t n, hipStream_t s) {
-  if (n == 0) return MADAM_OK;
-  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
-  return MADAM_OK;
-}
-template <class T>
-int down(T* dst, const T* src, size_t n, hipStream_t s) {
-  if (n == 0) return MADAM_OK;
-  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
-  return MADAM_OK;
-}
-
-// Fill view v from slot sl (device pointers) for a node of order n.
-void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
-  memset(&v, 0, sizeof v);
-  v.n = n;
-  v.m = cs.m;
-  v.nact = cs.nact;
-  v.L = sl.L.p;
-  v.X = sl.X.p;
-  v.Q = sl.Q.p;
-  v.W = sl.W.p;
-  v.V = sl.V.p;
-  v.S1 = sl.S1.p;
-  v.S2 = sl.S2.p;
-  v.ev = sl.ev.p;
-  v.evp = sl.evp.p;
-  v.y = sl.y.p;
-  v.cptr = sl.cptr.p;
-  v.cent = sl.cent.p;
-  v.cact = sl.cact.p;
-  v.cscale = sl.cscale.p;
-  v.aptr = sl.aptr.p;
-  v.aent = sl.aent.p;
-  v.apair = sl.apair.p;
-  v.p2a = sl.p2a.p;
-  v.t = sl.t.p;
-  v.s = sl.s.p;
-  v.u = sl.u.p;
-  v.rhs = sl.rhs.p;
-  v.cg_r = sl.r.p;
-  v.cg_p = sl.p.p;
-  v.cg_q = sl.q.p;
-  v.cg_w = sl.w.p;
-}
-
-int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {
+int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)hv.size();
   int rc;
-  if ((rc = up(sl.cptr.p, cs.cptr.data(), cs.cptr.size(), s)) ||
-      (rc = up(sl.cent.p, cs.cent.data(), cs.cent.size(), s)) ||
-      (rc = up(sl.cact.p, cs.cact.data(), cs.cact.size(), s)) ||
-      (rc = up(sl.aptr.p, cs.aptr.data(), cs.aptr.size(), s)) ||
-      (rc = up(sl.aent.p, cs.aent.data(), cs.aent.size(), s)) ||
-      (rc = up(sl.apair.p, cs.apair.data(), cs.apair.size(), s)) ||
-      (rc = up(sl.p2a.p, cs.p2a.data(), cs.p2a.size(), s)) ||
-      (rc = up(sl.cscale.p, cs.cscale.data(), cs.cscale.size(), s)))
-    return rc;
+  if ((rc = E.views.ensure(k))) return rc;
+  if ((rc = up(E.views.p, hv.data(), k, st))) return rc;
+  const int chunk = std::max(1, o.check_every);
+  int itmax_all = 0;
+  for (int b = 0; b < k; ++b) itmax_all = std::max(itmax_all, hv[b].itermax);
+  long launched = 0;
+  for (;;) {
+    if (launched > (long)itmax_all + 2 * chunk + 8) {
+      set_err("ADMM stop flags never set after %ld iterations", launched);
+      return MADAM_E_HIP;
+    }
+    launched += chunk;
+    for (int it = 0; it < chunk; ++it) {
+      TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
+      TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
+      TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
+      TIMED(PH_EIG, st,
+            madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1));
+      TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, st));
+    }
+    HIP_TRY(hipGetLastError());
+    if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    E.timer.flush();
+    bool all = true;
+    for (int b = 0; b < k; ++b) all = all && hv[b].done;
+    if (all) break;
+  }
+  TIMED(PH_SAFE, st, {
+    madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
+    madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
+    madam::launch_safe_bound(E.views.p, k, st);
+  });
+  HIP_TRY(hipGetLastError());
+  if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  E.timer.flush();
   return MADAM_OK;
 }
-
-int ensure_slots(int k) {
-  if ((int)g_eng->slots.size() < k) g_eng->slots.resize(k);
-  g_eng->hviews.resize(std::max<size_t>(g_eng->hviews.size(), k));
-  return g_eng->views.ensure(k);
-}
-
-int sync_views_down(int k) {
-  int rc = down(g_eng->hviews.data(), g_eng->views.p, k, g_eng->stream);
-  if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(g_eng->stream));
-  return MADAM_OK;
-}
-
-}  // namespace
 
 // =================================================================== C ABI
 
@@ -414,6 +132,14 @@ void madam_close(void) {
   if (!g_eng) return;
   (void)hipStreamSynchronize(g_eng->stream);
   for (auto& s : g_eng->slots) s.release();
+  for (auto& s : g_eng->sep) s.release();
+  g_eng->sepviews.release();
+  g_eng->sajobs.release();
+  g_eng->gwnodes.release();
+  g_eng->saout.release();
+  g_eng->sumx.release();
+  g_eng->xptrs.release();
+  g_eng->nvals.release();
   g_eng->views.release();
   g_eng->tmpd.release();
   (void)hipStreamDestroy(g_eng->stream);
@@ -471,43 +197,11 @@ int madam_admm_batch(int k, madam_admm_job* jobs, const madam_opts* opts_in) {
     v.scaleSigma = J.scaleSigma;
     v.itermax = J.itermax;
   }
-  if ((rc = up(E.views.p, E.hviews.data(), k, st))) return rc;
-
-  // ---- the ADMM loop (admm.c:520-704), all nodes together
-  const int chunk = std::max(1, o.check_every);
-  int itmax_all = 0;
-  for (int b = 0; b < k; ++b) itmax_all = std::max(itmax_all, jobs[b].itermax);
-  long launched = 0;
-  for (;;) {
-    if (launched > (long)itmax_all + 2 * chunk + 8) {
-      set_err("ADMM stop flags never set after %ld iterations", launched);
-      return MADAM_E_HIP;
-    }
-    launched += chunk;
-    for (int it = 0; it < chunk; ++it) {
-      TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
-      TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
-      TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
-      TIMED(PH_EIG, st,
-            madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st));
-      TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, st));
-    }
-    HIP_TRY(hipGetLastError());
-    if ((rc = sync_views_down(k))) return rc;
-    E.timer.flush();
-    bool all = true;
-    for (int b = 0; b < k; ++b) all = all && E.hviews[b].done;
-    if (all) break;
+  {
+    std::vector<NodeView> hv(E.hviews.begin(), E.hviews.begin() + k);
+    if ((rc = admm_loop(hv, o, nmax, mmax))) return rc;
+    std::copy(hv.begin(), hv.end(), E.hviews.begin());
   }
-  // ---- safe bound (admm.c:724-763)
-  TIMED(PH_SAFE, st, {
-    madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
-    madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st);
-    madam::launch_safe_bound(E.views.p, k, st);
-  });
-  HIP_TRY(hipGetLastError());
-  if ((rc = sync_views_down(k))) return rc;
-  E.timer.flush();
 
   // ---- results
   std::vector<double> evh, Vh;
@@ -616,7 +310,7 @@ int madam_project_sdp(int n, const double* W, double* X, double* Q, double* Z, i
   const hipStream_t st = g_eng->stream;
   const size_t nn = (size_t)n * n;
   if ((rc = up(sl.W.p, W, nn, st))) return rc;
-  madam::launch_jacobi(g_eng->views.p, 1, n, 0, 60, 1, g_eng->lds_limit, st);
+  madam::launch_jacobi(g_eng->views.p, 1, n, 0, 60, 1, g_eng->lds_limit, st, 1);
   madam::launch_project(g_eng->views.p, 1, n, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;
@@ -646,11 +340,153 @@ int madam_lambda_max(int n, const double* A, double* lam) {
   if ((rc = up(sl.W.p, A, (size_t)n * n, st))) return rc;
   // safe bound with y = 0 and no cuts: f = n * lambda_max
   HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
-  madam::launch_jacobi(g_eng->views.p, 1, n, 1, 160, 1, g_eng->lds_limit, st);
+  madam::launch_jacobi(g_eng->views.p, 1, n, 1, 160, 1, g_eng->lds_limit, st, 0);
   madam::launch_safe_bound(g_eng->views.p, 1, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;
   *lam = g_eng->hviews[0].lam;
+  return MADAM_OK;
+}
+
+
+// Triangle separation on node slots [0, k): X of each slot must be current.
+// Lists come back in enumeration order (type, i, j, k ascending).
+}  // extern "C"
+int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double thresh,
+                           std::vector<std::vector<madam_tri>>& lists, std::vector<double>& maxval) {
+  Engine& E = *g_eng;
+  const int k = (int)slots.size();
+  const hipStream_t st = E.stream;
+  int rc;
+  if ((int)E.sep.size() < k) E.sep.resize(k);
+  if ((rc = E.sepviews.ensure(k))) return rc;
+  std::vector<madam::SepView> hv(k);
+  int nmax = 2;
+  for (int b = 0; b < k; ++b) {
+    SepSlot& sp = E.sep[b];
+    if ((rc = sp.ensure(std::max(K[b], 1)))) return rc;
+    HIP_TRY(hipMemsetAsync(sp.ctl.p, 0, madam::CTL_N * sizeof(uint32_t), st));
+    HIP_TRY(hipMemsetAsync(sp.hist.p, 0, madam::SEP_NBIN * sizeof(uint32_t), st));
+    HIP_TRY(hipMemsetAsync(sp.maxbits.p, 0, sizeof(unsigned long long), st));
+    madam::SepView& v = hv[b];
+    v.n = E.hviews[slots[b]].n;
+    v.K = K[b];
+    v.thresh = thresh;
+    v.X = E.slots[slots[b]].X.p;
+    v.ctl = sp.ctl.p;
+    v.maxbits = sp.maxbits.p;
+    v.hist = sp.hist.p;
+    v.okey = sp.okey.p;
+    v.bkey = sp.bkey.p;
+    v.bval = sp.bval.p;
+    v.fkey = sp.fkey.p;
+    v.fval = sp.fval.p;
+    nmax = std::max(nmax, v.n);
+  }
+  if ((rc = up(E.sepviews.p, hv.data(), k, st))) return rc;
+  TIMED(PH_TRISEP, st, madam::launch_tri_separate(E.sepviews.p, k, nmax, st));
+  HIP_TRY(hipGetLastError());
+  lists.assign(k, {});
+  maxval.assign(k, 0.0);
+  std::vector<uint32_t> ctl(madam::CTL_N), keys;
+  std::vector<double> vals;
+  for (int b = 0; b < k; ++b) {
+    SepSlot& sp = E.sep[b];
+    unsigned long long mb = 0;
+    if ((rc = down(ctl.data(), sp.ctl.p, madam::CTL_N, st)) || (rc = down(&mb, sp.maxbits.p, 1, st)))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint32_t nf = ctl[madam::CTL_NFINAL];
+    keys.resize(nf);
+    vals.resize(nf);
+    if ((rc = down(keys.data(), sp.fkey.p, nf, st)) || (rc = down(vals.data(), sp.fval.p, nf, st)))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    const unsigned long long bits =
+        (mb & 0x8000000000000000ull) ? (mb & 0x7fffffffffffffffull) : ~mb;
+    double mx;
+    memcpy(&mx, &bits, sizeof mx);
+    maxval[b] = mx;
+    auto& L = lists[b];
+    L.resize(nf);
+    for (uint32_t q = 0; q < nf; ++q) {
+      const uint32_t key = keys[q];
+      L[q].type = (int)(key >> 30) + 1;
+      L[q].i = (key >> 20) & 1023;
+      L[q].j = (key >> 10) & 1023;
+      L[q].k = key & 1023;
+      L[q].value = vals[q];
+      L[q].y = 0.0;
+    }
+  }
+  E.timer.flush();
+  return MADAM_OK;
+}
+
+extern "C" {
+int madam_tri_separate(int n, const double* X, int K, double thresh, madam_tri* list, int* size,
+                       double* maxval) {
+  CutSet cs;
+  int rc = one_node(n, 0, nullptr, 0, nullptr, 0, nullptr, cs);
+  if (rc) return rc;
+  if (K < 0 || !list || !size) {
+    set_err("madam_tri_separate: bad arguments");
+    return MADAM_E_ARG;
+  }
+  if ((rc = up(g_eng->slots[0].X.p, X, (size_t)n * n, g_eng->stream))) return rc;
+  std::vector<std::vector<madam_tri>> lists;
+  std::vector<double> mx;
+  if ((rc = tri_separate_slots(std::vector<int>{0}, &K, thresh, lists, mx))) return rc;
+  *size = (int)lists[0].size();
+  std::copy(lists[0].begin(), lists[0].end(), list);
+  if (maxval) *maxval = mx[0];
+  return MADAM_OK;
+}
+
+// Simulated-annealing runs on one X (host entry for tests and tools).
+int madam_qap_sa(int n, const double* X, int k, int nruns, const int* types,
+                 const long long* stream_ids, unsigned long long seed, double* cost, int* perms,
+                 long long* draws) {
+  CutSet cs;
+  int rc = one_node(n, 0, nullptr, 0, nullptr, 0, nullptr, cs);
+  if (rc) return rc;
+  if ((k != 5 && k != 7) || nruns <= 0) {
+    set_err("madam_qap_sa: k=%d nruns=%d", k, nruns);
+    return MADAM_E_ARG;
+  }
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  Slot& sl = E.slots[0];
+  if ((rc = up(sl.X.p, X, (size_t)n * n, st))) return rc;
+  if ((rc = E.sumx.ensure(1)) || (rc = E.xptrs.ensure(1)) || (rc = E.nvals.ensure(1)) ||
+      (rc = E.sajobs.ensure(nruns)) || (rc = E.saout.ensure(nruns)))
+    return rc;
+  const double* xp = sl.X.p;
+  if ((rc = up(E.xptrs.p, &xp, 1, st)) || (rc = up(E.nvals.p, &n, 1, st))) return rc;
+  madam::launch_sumx(E.xptrs.p, E.nvals.p, E.sumx.p, 1, st);
+  std::vector<madam::SaJob> jobs(nruns);
+  for (int r = 0; r < nruns; ++r) {
+    jobs[r].X = sl.X.p;
+    jobs[r].sumX = E.sumx.p;
+    jobs[r].n = n;
+    jobs[r].k = k;
+    jobs[r].neg = types[r] - 1;
+    jobs[r].nstride = madam::sa_nstride(n);
+    jobs[r].seed = seed;
+    jobs[r].stream_id = stream_ids[r];
+  }
+  if ((rc = up(E.sajobs.p, jobs.data(), nruns, st))) return rc;
+  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, nruns, n, E.saout.p, st));
+  HIP_TRY(hipGetLastError());
+  std::vector<madam::SaOut> out(nruns);
+  if ((rc = down(out.data(), E.saout.p, nruns, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  E.timer.flush();
+  for (int r = 0; r < nruns; ++r) {
+    cost[r] = out[r].cost;
+    for (int a = 0; a < k; ++a) perms[(size_t)r * k + a] = out[r].perm[a];
+    if (draws) draws[r] = out[r].draws;
+  }
   return MADAM_OK;
 }
 

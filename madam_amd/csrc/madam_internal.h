@@ -30,6 +30,9 @@ struct NodeView {
   double *L, *X, *Q, *W, *V, *ev, *y;  // n*n (L, X, Q, W, V), n (ev, y)
   double *S1, *S2;                     // n*n scratch
   double *evp;                         // n: kept eigenvalues of the latest projection (0 if dropped)
+  double *ip1, *ip2, *ip3;             // n*n IPM scratch
+  double *ipv;                         // 4n IPM vectors
+  double *ipP;                         // n(n+1)/2 packed Cholesky factor (when not in LDS)
 
   const int *cptr;      // m+1
   const int *cent;      // cptr[m]: lo or ~lo
@@ -52,7 +55,9 @@ struct NodeView {
   int itermax, nbit, done, M;
   int cg_iters, cg_total, sweeps_last, sweeps_total;
   int eig_warm;   // V holds a usable basis from the previous projection
-  int pad1;
+  int ipm_iters;  // IPM iterations of the last k_ipm
+  int ipm_status; // 0 ok, else the reference's failure point (1..4)
+  int pad2;
 };
 
 // Launch helpers implemented in madam_device.hip.  All take the device array
@@ -61,7 +66,8 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
 void launch_cg(NodeView* nodes, int nb, double rtol, int maxit, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
-                   int ignore_done, size_t lds_limit, hipStream_t stream);
+                   int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
+size_t jacobi_lds_bytes(int nmax, bool with_v);
 void launch_admm_post(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_safe_bound(NodeView* nodes, int nb, hipStream_t stream);
 void launch_op_B(const NodeView* nodes, int nb, const double* Xin, double* yout, int mmax,
@@ -69,5 +75,6 @@ void launch_op_B(const NodeView* nodes, int nb, const double* Xin, double* yout,
 void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout, int nmax,
                   hipStream_t stream);
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t stream);
 
 }  // namespace madam

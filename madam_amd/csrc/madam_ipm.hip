@@ -1,0 +1,285 @@
+// madam_ipm.hip -- basic-SDP primal-dual predictor-corrector interior-point
+// method of ipm_mc_pk.c:28-347 (max <L,X> s.t. diag(X) = e, X psd), one
+// persistent workgroup per node running every iteration on the device.
+//
+// Per node: X (nd.X), Z = Diag(y) - L (nd.Q), y (nd.y); scratch Zi (S1),
+// M (S2), dX (W), dX1 (ip1), T1 (ip2), T2 (ip3), vectors in ipv.  Cholesky
+// factors are packed lower triangles, in LDS when they fit, else in ipP.
+// Where the reference exits (ipm_mc_pk.c:124-135, 161-165, 218-222) the
+// kernel stops with ipm_status = 1..4 and the host reports the failure.
+#include "madam_devutil.h"
+#include "madam_internal.h"
+
+namespace madam {
+
+// Factor the packed lower triangle P (order n, leading order n2) in place:
+// P = L with L L^T = A.  Returns 0 or j+1 at the first non-positive pivot.
+__device__ int chol_packed(double* P, int n, int n2, int* flag) {
+  const int tid = threadIdx.x;
+  for (int j = 0; j < n; ++j) {
+    const double djj = P[pk(j, j, n2)];
+    if (!(djj > 0.0)) return j + 1;  // uniform: every thread read the same value
+    const double d = sqrt(djj);
+    __syncthreads();
+    if (tid == 0) P[pk(j, j, n2)] = d;
+    const int base = pk(j, j, n2);
+    for (int i = j + 1 + tid; i < n; i += blockDim.x) P[base + (i - j)] /= d;
+    __syncthreads();
+    const int m = n - j - 1;
+    const int npair = m * (m + 1) / 2;
+    for (int f = tid; f < npair; f += blockDim.x) {
+      int a = (int)((sqrtf(8.0f * (float)f + 1.0f) - 1.0f) * 0.5f);
+      if ((a + 1) * (a + 2) / 2 <= f) ++a;
+      if (a * (a + 1) / 2 > f) --a;
+      const int b = f - a * (a + 1) / 2;
+      const int i = j + 1 + a, k = j + 1 + b;
+      P[pk(i, k, n2)] -= P[base + (i - j)] * P[base + (k - j)];
+    }
+    __syncthreads();
+  }
+  (void)flag;
+  return 0;
+}
+
+// P[lower] = A + alpha * Diag(dv) (dv may be null) or A + alpha * B.
+__device__ void load_packed(double* P, int n, int n2, const double* A, double alpha,
+                            const double* B, const double* dv) {
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int j = f / n, i = f - j * n;
+    if (i < j) continue;
+    double v = A[f];
+    if (B) v += alpha * B[f];
+    if (dv && i == j) v += alpha * dv[i];
+    P[pk(i, j, n2)] = v;
+  }
+  __syncthreads();
+}
+
+// Solve (L L^T) x = b with the packed factor; wave 0 does the work, b and x
+// may alias.  Row-oriented sweeps with wave reductions.
+__device__ void chol_solve1(const double* P, int n, int n2, const double* b, double* x) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    for (int i = 0; i < n; ++i) {
+      double part = 0.0;
+      for (int k = lane; k < i; k += 64) part += P[pk(i, k, n2)] * x[k];
+      const double sacc = wave_sum(part);
+      const double xi = (b[i] - sacc) / P[pk(i, i, n2)];
+      if (lane == (i & 63)) x[i] = xi;
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double part = 0.0;
+      for (int k = i + 1 + lane; k < n; k += 64) part += P[pk(k, i, n2)] * x[k];
+      const double sacc = wave_sum(part);
+      // x[i] written only by lane (i & 63), read back by the same lane
+      double xi = 0.0;
+      if (lane == (i & 63)) xi = (x[i] - sacc) / P[pk(i, i, n2)];
+      if (lane == (i & 63)) x[i] = xi;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __syncthreads();
+}
+
+struct IpmOut {
+  double* C;
+  int n;
+  __device__ void operator()(int i, int j, double v) const { C[i + (size_t)j * n] = v; }
+};
+
+template <bool P_LDS>
+__global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
+  extern __shared__ double smem[];
+  NodeView& nd = nodes[blockIdx.x];
+  const int n = nd.n, tid = threadIdx.x;
+  const int n2 = n;
+  double* red = smem;
+  double* tA = red + 64;
+  double* tB = tA + 16 * 65;
+  __shared__ int flag;
+  double* P = P_LDS ? (tB + 16 * 65) : nd.ipP;
+  const double* __restrict__ L = nd.L;
+  double* X = nd.X;
+  double* Z = nd.Q;
+  double* y = nd.y;
+  double* Zi = nd.S1;
+  double* Mm = nd.S2;
+  double* dX = nd.W;
+  double* dX1 = nd.ip1;
+  double* T1 = nd.ip2;
+  double* T2 = nd.ip3;
+  double* dy = nd.ipv;
+  double* dy1 = nd.ipv + n;
+  double* dd = nd.ipv + 2 * n;
+  const size_t nn = (size_t)n * n;
+
+  // y = 1.1 + column sums of |L| (ipm_mc_pk.c:61-68); X = I; Z = Diag(y) - L
+  for (int i = tid; i < n; i += WG) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += fabs(L[j + (size_t)i * n]);
+    y[i] = acc + 1.1;
+  }
+  __syncthreads();
+  for (size_t f = tid; f < nn; f += WG) {
+    const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+    X[f] = (i == j) ? 1.0 : 0.0;
+    Z[f] = ((i == j) ? y[i] : 0.0) - L[f];
+  }
+  __syncthreads();
+  double v3[3] = {0, 0, 0};  // phi, psi, <Z,X>
+  for (int i = tid; i < n; i += WG) v3[0] += y[i];
+  for (size_t f = tid; f < nn; f += WG) {
+    v3[1] += L[f] * X[f];
+    v3[2] += Z[f] * X[f];
+  }
+  block_sums<3>(v3, red);
+  double phi = v3[0], psi = v3[1], mu = v3[2] / (2.0 * n);
+  int it = 0, status = 0;
+  while (phi - psi > 1e-4 && it < maxit) {
+    ++it;
+    // ---- Zi = Z^{-1}: factor, then solve L L^T Zi = I column by column
+    load_packed(P, n, n2, Z, 0.0, nullptr, nullptr);
+    if (chol_packed(P, n, n2, &flag)) { status = 1; break; }
+    for (int c = tid; c < n; c += WG) {
+      double* col = Zi + (size_t)c * n;
+      for (int i = 0; i < c; ++i) col[i] = 0.0;
+      for (int i = c; i < n; ++i) {
+        double acc = (i == c) ? 1.0 : 0.0;
+        for (int k = c; k < i; ++k) acc -= P[pk(i, k, n2)] * col[k];
+        col[i] = acc / P[pk(i, i, n2)];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double acc = col[i];
+        for (int k = i + 1; k < n; ++k) acc -= P[pk(k, i, n2)] * col[k];
+        col[i] = acc / P[pk(i, i, n2)];
+      }
+    }
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) {  // symmetric from the lower triangle
+      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+      if (i < j) Zi[f] = Zi[j + (size_t)i * n];
+    }
+    __syncthreads();
+    // ---- predictor: dy1 = (Zi .* X) \ (-e)
+    for (size_t f = tid; f < nn; f += WG) Mm[f] = Zi[f] * X[f];
+    for (int i = tid; i < n; i += WG) dy1[i] = -1.0;
+    __syncthreads();
+    load_packed(P, n, n2, Mm, 0.0, nullptr, nullptr);
+    if (chol_packed(P, n, n2, &flag)) { status = 3; break; }
+    chol_solve1(P, n, n2, dy1, dy1);
+    // dX1 = -Zi Diag(dy1) X - X, symmetrised
+    for (size_t f = tid; f < nn; f += WG) {
+      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+      T1[f] = -dy1[i] * X[f];
+    }
+    __syncthreads();
+    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX1, n}, tA, tB);
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) dX1[f] -= X[f];
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) {
+      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+      if (i <= j) {
+        const size_t g = j + (size_t)i * n;
+        const double a = 0.5 * (dX1[f] + dX1[g]);
+        T2[f] = a;
+        T2[g] = a;
+      }
+    }
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) dX1[f] = T2[f];
+    __syncthreads();
+    // ---- corrector: dy2 = (Zi .* X) \ (mu diag(Zi) - (Zi .* dX1) dy1)
+    for (size_t f = tid; f < nn; f += WG) T1[f] = Zi[f] * dX1[f];
+    __syncthreads();
+    for (int i = tid; i < n; i += WG) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) acc += T1[i + (size_t)j * n] * dy1[j];
+      dy[i] = -acc + mu * Zi[(size_t)i * (n + 1)];
+    }
+    __syncthreads();
+    load_packed(P, n, n2, Mm, 0.0, nullptr, nullptr);
+    if (chol_packed(P, n, n2, &flag)) { status = 4; break; }
+    chol_solve1(P, n, n2, dy, dy);
+    // dX2 = mu Zi - Zi (Diag(dy2) X + Diag(dy1) dX1)
+    for (size_t f = tid; f < nn; f += WG) {
+      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+      T1[f] = -dy[i] * X[f] + (-dy1[i] * dX1[f]);
+    }
+    __syncthreads();
+    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX, n}, tA, tB);
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) dX[f] += mu * Zi[f] + dX1[f];
+    for (int i = tid; i < n; i += WG) dy[i] += dy1[i];
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) {
+      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+      if (i <= j) {
+        const size_t g = j + (size_t)i * n;
+        const double a = 0.5 * (dX[f] + dX[g]);
+        T2[f] = a;
+        T2[g] = a;
+      }
+    }
+    __syncthreads();
+    for (size_t f = tid; f < nn; f += WG) dX[f] = T2[f];
+    __syncthreads();
+    // ---- step lengths by Cholesky back-off (ipm_mc_pk.c:273-301)
+    double ap = 1.0;
+    for (int tries = 0; tries < 400; ++tries) {
+      load_packed(P, n, n2, X, ap, dX, nullptr);
+      if (!chol_packed(P, n, n2, &flag)) break;
+      ap *= 0.8;
+    }
+    if (ap < 1.0) ap *= 0.95;
+    double ad = 1.0;
+    for (int tries = 0; tries < 400; ++tries) {
+      load_packed(P, n, n2, Z, ad, nullptr, dy);
+      if (!chol_packed(P, n, n2, &flag)) break;
+      ad *= 0.8;
+    }
+    if (ad < 1.0) ad *= 0.95;
+    // ---- update
+    for (size_t f = tid; f < nn; f += WG) X[f] += ap * dX[f];
+    for (int i = tid; i < n; i += WG) {
+      y[i] += ad * dy[i];
+      Z[(size_t)i * (n + 1)] += ad * dy[i];
+    }
+    __syncthreads();
+    double w3[3] = {0, 0, 0};
+    for (size_t f = tid; f < nn; f += WG) {
+      w3[0] += Z[f] * X[f];
+      w3[1] += L[f] * X[f];
+    }
+    for (int i = tid; i < n; i += WG) w3[2] += y[i];
+    block_sums<3>(w3, red);
+    mu = w3[0] / (2.0 * n);
+    if (ap + ad > 1.6) mu *= 0.5;
+    if (ap + ad > 1.9) mu *= 0.2;
+    phi = w3[2];
+    psi = w3[1];
+  }
+  (void)dd;
+  if (tid == 0) {
+    nd.f = phi;
+    nd.ipm_iters = it;
+    nd.ipm_status = status;
+    nd.eig_warm = 0;
+  }
+}
+
+size_t ipm_lds_bytes(int n, bool packed) {
+  size_t d = 64 + 2 * 16 * 65;
+  if (packed) d += (size_t)n * (n + 1) / 2;
+  return d * sizeof(double);
+}
+
+void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t st) {
+  if (ipm_lds_bytes(nmax, true) <= lds_limit)
+    k_ipm<true><<<nb, WG, ipm_lds_bytes(nmax, true), st>>>(nodes, 500);
+  else
+    k_ipm<false><<<nb, WG, ipm_lds_bytes(nmax, false), st>>>(nodes, 500);
+}
+
+}  // namespace madam

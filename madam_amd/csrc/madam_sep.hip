@@ -1,0 +1,355 @@
+// madam_sep.hip -- cutting-plane separation and primal heuristic kernels
+// (cutting_planes.c:20-127, qap_simulated_annealing.c:12-206,
+// heuristic.c:24-201) for gfx950.
+//
+// Triangle separation (getViolated_TriangleInequalities) enumerates the
+// 4 * C(n,3) rows B_c(X) - 1 of every node in parallel: one workgroup per
+// (node, i) handles all j < i, k < j with row i of X staged in LDS and the
+// (j,k) entries read as contiguous columns of the symmetric X.  The top-K
+// selection is exact: pass 0 counts violated rows (> thresh) and takes the
+// maximum, pass 1 histograms the violated values over [thresh, max], a scan
+// finds the bin holding the K-th largest value, pass 2 emits every row above
+// that bin and collects the boundary bin, and a last kernel ranks the
+// boundary rows by (value desc, enumeration order asc), keeps what is
+// needed and sorts the list into enumeration order.  Keys pack
+// (type-1, i, j, k) as 2+10+10+10 bits, so key order is the reference's
+// enumeration order (type outermost, then i, j, k ascending).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "madam_sep.h"
+
+namespace madam {
+
+constexpr int SEP_WG = 256;
+
+__device__ __forceinline__ unsigned long long ord_bits(double v) {
+  unsigned long long b = __double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_ord(unsigned long long o) {
+  unsigned long long b = (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
+  return __longlong_as_double(b);
+}
+
+// the four triangle rows minus one, in the reference's operation order
+// (cutting_planes.c:24-36)
+__device__ __forceinline__ double tri_val(int type, double a, double b, double c) {
+#pragma clang fp contract(off)
+  switch (type) {
+    case 1: return -a - b - c - 1.0;
+    case 2: return -a + b + c - 1.0;
+    case 3: return a - b + c - 1.0;
+    default: return a + b - c - 1.0;
+  }
+}
+
+__device__ __forceinline__ uint32_t tri_key(int type, int i, int j, int k) {
+  return ((uint32_t)(type - 1) << 30) | ((uint32_t)i << 20) | ((uint32_t)j << 10) | (uint32_t)k;
+}
+
+__device__ __forceinline__ void pair_decode(int p, int& j, int& k) {
+  // p = j(j-1)/2 + k, 0 <= k < j
+  int jj = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)p)) * 0.5f);
+  if (jj * (jj - 1) / 2 > p) --jj;
+  if ((jj + 1) * jj / 2 <= p) ++jj;
+  j = jj;
+  k = p - jj * (jj - 1) / 2;
+}
+
+// PASS 0: count + max; PASS 1: histogram; PASS 2: emit.
+template <int PASS>
+__global__ __launch_bounds__(SEP_WG) void k_tri_pass(SepView* sv) {
+  __shared__ double xi[1024];
+  __shared__ uint32_t hist[SEP_NBIN];
+  __shared__ unsigned long long smax;
+  __shared__ uint32_t scount;
+  SepView& s = sv[blockIdx.y];
+  const int n = s.n, i = blockIdx.x;
+  if (i >= n || i < 2) return;
+  if (PASS > 0 && s.ctl[CTL_COUNT] <= (uint32_t)s.K) {
+    if (PASS == 1) return;
+  }
+  const double* __restrict__ X = s.X;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < i; j += SEP_WG) xi[j] = X[i + (size_t)j * n];
+  if (PASS == 1)
+    for (int b = tid; b < SEP_NBIN; b += SEP_WG) hist[b] = 0;
+  if (tid == 0) {
+    smax = 0;
+    scount = 0;
+  }
+  __syncthreads();
+  const double thr = s.thresh;
+  double lo = 0.0, scale = 0.0;
+  int emit_all = 0, bstar = 0;
+  if (PASS >= 1) {
+    const double vmax = from_ord(s.maxbits[0]);
+    lo = thr;
+    scale = (vmax > thr) ? (double)SEP_NBIN / (vmax - thr) : 0.0;
+    emit_all = s.ctl[CTL_COUNT] <= (uint32_t)s.K;
+    bstar = (int)s.ctl[CTL_BSTAR];
+  }
+  double vmax_loc = -1e300;
+  uint32_t cnt_loc = 0;
+  const int npairs = i * (i - 1) / 2;
+  for (int p = tid; p < npairs; p += SEP_WG) {
+    int j, k;
+    pair_decode(p, j, k);
+    const double a = xi[j], b = xi[k], c = X[k + (size_t)j * n];
+#pragma unroll
+    for (int type = 1; type <= 4; ++type) {
+      const double v = tri_val(type, a, b, c);
+      if (PASS == 0) {
+        vmax_loc = fmax(vmax_loc, v);
+        cnt_loc += v > thr;
+      } else if (v > thr) {
+        int bin = (int)((v - lo) * scale);
+        bin = bin < 0 ? 0 : (bin >= SEP_NBIN ? SEP_NBIN - 1 : bin);
+        if (PASS == 1) {
+          atomicAdd(&hist[bin], 1u);
+        } else {
+          const uint32_t key = tri_key(type, i, j, k);
+          if (emit_all || bin > bstar) {
+            const uint32_t slot = atomicAdd(&s.ctl[CTL_NOUT], 1u);
+            if (slot < (uint32_t)s.K) s.okey[slot] = key;
+          } else if (bin == bstar) {
+            const uint32_t slot = atomicAdd(&s.ctl[CTL_NBOUND], 1u);
+            if (slot < (uint32_t)SEP_BCAP) {
+              s.bkey[slot] = key;
+              s.bval[slot] = v;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (PASS == 0) {
+    atomicMax(&smax, ord_bits(vmax_loc));
+    atomicAdd(&scount, cnt_loc);
+    __syncthreads();
+    if (tid == 0) {
+      atomicMax(&s.maxbits[0], smax);
+      atomicAdd(&s.ctl[CTL_COUNT], scount);
+    }
+  } else if (PASS == 1) {
+    __syncthreads();
+    for (int b = tid; b < SEP_NBIN; b += SEP_WG)
+      if (hist[b]) atomicAdd(&s.hist[b], hist[b]);
+  }
+}
+
+// One workgroup per node: find the bin holding the K-th largest violated value.
+__global__ __launch_bounds__(SEP_WG) void k_tri_scan(SepView* sv) {
+  SepView& s = sv[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  const uint32_t count = s.ctl[CTL_COUNT];
+  if (count <= (uint32_t)s.K) {
+    s.ctl[CTL_BSTAR] = 0;
+    s.ctl[CTL_NEED] = 0;
+    return;
+  }
+  uint32_t above = 0;
+  int b = SEP_NBIN - 1;
+  for (; b >= 0; --b) {
+    if (above + s.hist[b] >= (uint32_t)s.K) break;
+    above += s.hist[b];
+  }
+  s.ctl[CTL_BSTAR] = (uint32_t)b;
+  s.ctl[CTL_NEED] = (uint32_t)s.K - above;
+}
+
+// One workgroup per node: finish the list.  Boundary rows are ranked by
+// (value desc, key asc) and the first `need` kept; the list is then written
+// in key (= enumeration) order with the row values recomputed from X.
+__global__ __launch_bounds__(1024) void k_tri_finish(SepView* sv) {
+  SepView& s = sv[blockIdx.x];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const uint32_t count = s.ctl[CTL_COUNT];
+  uint32_t nout = s.ctl[CTL_NOUT];
+  if (nout > (uint32_t)s.K) nout = s.K;
+  if (count > (uint32_t)s.K) {
+    uint32_t nb = s.ctl[CTL_NBOUND];
+    if (nb > SEP_BCAP) nb = SEP_BCAP;
+    const uint32_t need = s.ctl[CTL_NEED];
+    for (uint32_t q = tid; q < nb; q += NT) {
+      const double v = s.bval[q];
+      const uint32_t key = s.bkey[q];
+      uint32_t rank = 0;
+      for (uint32_t r = 0; r < nb; ++r) {
+        const double w = s.bval[r];
+        rank += (w > v) || (w == v && s.bkey[r] < key);
+      }
+      if (rank < need) s.okey[nout + rank] = key;
+    }
+    __syncthreads();
+    nout += need < nb ? need : nb;
+    if (nout > (uint32_t)s.K) nout = s.K;
+  }
+  // sort keys ascending by rank counting (keys are distinct)
+  const int n = s.n;
+  for (uint32_t q = tid; q < nout; q += NT) {
+    const uint32_t key = s.okey[q];
+    uint32_t rank = 0;
+    for (uint32_t r = 0; r < nout; ++r) rank += s.okey[r] < key;
+    const int type = (int)(key >> 30) + 1, i = (key >> 20) & 1023, j = (key >> 10) & 1023,
+              k = key & 1023;
+    s.fkey[rank] = key;
+    s.fval[rank] = tri_val(type, s.X[i + (size_t)j * n], s.X[i + (size_t)k * n],
+                           s.X[j + (size_t)k * n]);
+  }
+  if (tid == 0) {
+    s.ctl[CTL_NFINAL] = nout;
+  }
+}
+
+void launch_tri_separate(SepView* sv, int nb, int nmax, hipStream_t st) {
+  dim3 grid(nmax, nb);
+  k_tri_pass<0><<<grid, SEP_WG, 0, st>>>(sv);
+  k_tri_pass<1><<<grid, SEP_WG, 0, st>>>(sv);
+  k_tri_scan<<<nb, SEP_WG, 0, st>>>(sv);
+  k_tri_pass<2><<<grid, SEP_WG, 0, st>>>(sv);
+  k_tri_finish<<<nb, 1024, 0, st>>>(sv);
+}
+
+// ------------------------------------------------------------------ SA
+//
+// qap_simulated_annealing (qap_simulated_annealing.c:12-206), one lane per
+// run.  Run r draws from its own stream rand31(seed, r, 0), (..., 1), ...
+// (splitmix64 based, values in [0, 2^31-1] like glibc rand()).  The
+// permutation lives in LDS (16-bit entries); X is read from HBM.  Same
+// arithmetic order as the reference (contraction off), so with the same
+// stream the run is identical draw for draw.
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ int rand31(uint64_t base, uint64_t& ctr) {
+  return (int)(splitmix64(base + ctr++) >> 33);
+}
+
+__global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int njobs, SaOut* out) {
+#pragma clang fp contract(off)
+  extern __shared__ unsigned short permbuf[];
+  const int r = blockIdx.x * SA_WG + threadIdx.x;
+  if (r >= njobs) return;
+  const SaJob J = jobs[r];
+  const int n = J.n, k = J.k;
+  const double* __restrict__ X = J.X;
+  unsigned short* perm = permbuf + (size_t)threadIdx.x * J.nstride;
+  uint64_t base = splitmix64(J.seed ^ splitmix64((uint64_t)J.stream_id));
+  uint64_t ctr = 0;
+  // H = e e^T with e_a = -1 for a < neg (cutting_planes.c:242-244, 438-441)
+#define SG(a) ((a) < J.neg ? -1 : 1)
+#define XV(a, b) X[(size_t)(a) + (size_t)(b) * n]
+  const double sumX = *J.sumX;
+  double temp0 = (double)(k * k) * sumX / (double)(n * (n - 1));
+  double cost = 1e9;
+  int best[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) perm[i] = (unsigned short)i;
+  for (int i = n - 1; i > 0; --i) {
+    const int rr = rand31(base, ctr) % i;
+    const unsigned short sw = perm[i];
+    perm[i] = perm[rr];
+    perm[rr] = sw;
+  }
+  double sol = 0.0;
+  for (int i = 0; i < k; ++i) {
+    sol += XV(perm[i], perm[i]);
+    for (int j = i + 1; j < k; ++j) sol += 2 * (SG(j) * SG(i)) * XV(perm[j], perm[i]);
+  }
+  double T = temp0;
+  int m1 = n;
+  int changed = 1;
+  long moves = 0;
+  while (changed > 0) {
+    changed = 0;
+    for (int it = 0; it < m1; ++it) {
+      int i1 = rand31(base, ctr) % k;
+      int i2 = rand31(base, ctr) % n;
+      if (i2 < i1) {
+        const int sw = i1;
+        i1 = i2;
+        i2 = sw;
+      }
+      const int p1 = perm[i1], p2 = perm[i2];
+      double delta = 0.0;
+      for (int i = 0; i < k; ++i) {
+        const int pi = perm[i];
+        delta += (SG(i) * SG(i1)) * (XV(pi, p2) - XV(pi, p1));
+      }
+      if (i2 < k)
+        for (int i = 0; i < k; ++i) {
+          const int pi = perm[i];
+          delta += (SG(i) * SG(i2)) * (XV(pi, p1) - XV(pi, p2));
+        }
+      delta *= 2.0;
+      const double d1 = XV(p1, p1) + XV(p2, p2) - 2.0 * XV(p2, p1);
+      double d2 = 1.0;
+      if (i2 < k) d2 += -2 * (SG(i1) * SG(i2)) + 1.0;
+      delta += d1 * d2;
+      const double cand = sol + delta;
+      int accept;
+      if (delta > 0) {
+        const double rt = delta / T;
+        if (rt > 5)
+          accept = 0;
+        else
+          accept = ((double)rand31(base, ctr) / ((double)2147483647)) < exp(-rt);
+      } else {
+        accept = 1;
+      }
+      if (accept) {
+        if (fabs(delta) > 0.0001) changed = 1;
+        perm[i1] = (unsigned short)p2;
+        perm[i2] = (unsigned short)p1;
+        sol = cand;
+        if (sol < cost) {
+          cost = sol;
+          for (int a = 0; a < k; ++a) best[a] = perm[a];
+        }
+      }
+      ++moves;
+    }
+    T *= 0.6;
+    m1 = (int)round(m1 * 1.1);
+  }
+#undef XV
+#undef SG
+  out[r].cost = cost;
+  for (int a = 0; a < 7; ++a) out[r].perm[a] = a < k ? best[a] : -1;
+  out[r].draws = (long long)ctr;
+  out[r].moves = moves;
+}
+
+// sum_i X_ii + 2 sum_{j>i} |X_ji| in the reference's order
+// (qap_simulated_annealing.c:28-35), one lane per node.
+__global__ void k_sumx(const double* const* Xs, const int* ns, double* out, int nb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double* X = Xs[b];
+  const int n = ns[b];
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) {
+    acc += X[i + (size_t)i * n];
+    for (int j = i + 1; j < n; ++j) acc += 2 * fabs(X[j + (size_t)i * n]);
+  }
+  out[b] = acc;
+}
+
+void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipStream_t st) {
+  k_sumx<<<(nb + 63) / 64, 64, 0, st>>>(X, n, out, nb);
+}
+
+void launch_qap_sa(const SaJob* jobs, int njobs, int nmax, SaOut* out, hipStream_t st) {
+  const int blocks = (njobs + SA_WG - 1) / SA_WG;
+  const size_t lds = (size_t)SA_WG * sa_nstride(nmax) * sizeof(unsigned short);
+  k_qap_sa<<<blocks, SA_WG, lds, st>>>(jobs, njobs, out);
+}
+
+}  // namespace madam

@@ -1,0 +1,105 @@
+"""GPU node evaluation and branch and bound against the oracle and brute force.
+
+  ipm_mc_pk      (ipm_mc_pk.c:28-347): phi within 1e-9 rel., X within 1e-6,
+                 same iteration count
+  GW + 1-opt     (heuristic.c:24-201): same hyperplanes -> identical 1-opt
+                 cuts, values equal to z'^T L z'
+  SDPbound       (bounding.c:25-250): root bound within 1e-6 rel. of the
+                 oracle at identical settings (triangle cuts; incumbent
+                 pinned to the optimum so the heuristics cannot steer the
+                 trajectory)
+  B&B            proven optimum equals brute force (n <= 18) and the oracle
+"""
+import itertools
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import madam_amd as G
+from madam_amd import instances as I
+from oracle import oracle_py as O
+from tests.helpers import graph_L
+
+pytestmark = pytest.mark.gpu
+
+
+def brute_force(SPL):
+    N = SPL.shape[0]
+    A = SPL[:N - 1, :N - 1]
+    X = np.array(list(itertools.product([0, 1], repeat=N - 1)), dtype=float)
+    vals = np.einsum("ij,jk,ik->i", X, A, X)
+    return vals.max()
+
+
+@pytest.mark.parametrize("spec", ["g05_60.0", "w01_100.3", "config2"])
+def test_ipm_matches_oracle(spec):
+    SPL = graph_L(I.config2(2) if spec == "config2" else spec)
+    L, _ = O.create_subproblem(SPL, np.zeros(SPL.shape[0] - 1, np.int32),
+                               np.zeros(SPL.shape[0] - 1, np.int32))
+    Xg, yg, Zg, phig, itg = G.ipm(L)
+    Xo, yo, Zo, phio, ito = O.ipm(L)
+    print(spec, "iters", itg, ito, "phi", phig, phio)
+    assert itg == ito
+    assert abs(phig - phio) <= 1e-9 * abs(phio)
+    assert np.abs(Xg - Xo).max() <= 1e-6
+    assert np.abs(yg - yo).max() <= 1e-6 * np.abs(yo).max()
+
+
+def test_gw_rounding_matches_oracle():
+    SPL = graph_L("g05_60.0")
+    L, _ = O.create_subproblem(SPL, np.zeros(59, np.int32), np.zeros(59, np.int32))
+    X, y, Z, phi, _ = O.ipm(L)
+    _, _, Zf, _ = O.project_psd(X)
+    num = 24
+    vals, cuts = G.gw_rounding(L, Zf, num, seed=77, stream=5)
+    planes = G.gw_planes(77, 5, num, Zf.shape[1])
+    cuts_o, _ = O.gw_explicit(L, Zf, planes)
+    assert np.array_equal(cuts.astype(np.int32), cuts_o)
+    for h in range(num):
+        z = cuts_o[h].astype(float)
+        z[-1] = 1.0
+        assert vals[h] == z @ L @ z
+
+
+@pytest.mark.parametrize("graph,opt", [("g05_60.0", 536.0), ("g05_60.1", None)])
+def test_sdpbound_root_matches_oracle(graph, opt):
+    SPL = graph_L(graph)
+    p = O.default_params(include_Pent=0, include_Hepta=0)
+    if opt is None:
+        s0 = O.Solver(SPL, p)
+        opt = s0.bab()
+    sol = O.Solver(SPL, O.default_params(include_Pent=0, include_Hepta=0))
+    sol.set_lb(opt)
+    bo, _ = sol.root_bound()
+    gp = G.default_params(include_Pent=0, include_Hepta=0)
+    B = G.Bounder(SPL, gp, LB=opt)
+    root = B.node()
+    bg = B.sdpbound([root], ranks=[0])[0]
+    print(graph, "gpu", bg, "oracle", bo, "admm iters gpu", B.ctx.admm_iters, "oracle", sol.s.admm_iters)
+    assert abs(bg - bo) <= 1e-6 * abs(bo)
+    assert bg >= opt
+
+
+@pytest.mark.parametrize("n,p,weights,seed", [(12, 0.5, [1], 1), (14, 0.6, [-1, 1], 2),
+                                              (16, 0.4, list(range(-10, 11)), 3),
+                                              (18, 0.5, [-1, 0, 1], 4)])
+def test_bab_matches_brute_force(n, p, weights, seed):
+    SPL = I.objective_matrix(*I.gnp(n, p, weights, seed))
+    best = brute_force(SPL)
+    B = G.Bounder(SPL, G.default_params(), batch=4)
+    opt = B.bab()
+    assert opt == best
+    x = np.array(B.ctx.best_x[:n - 1], dtype=float)
+    assert x @ SPL[:n - 1, :n - 1] @ x == best
+    assert B.ctx.root_bound >= best - 1e-9
+
+
+def test_bab_g05_60_optimum():
+    SPL = graph_L("g05_60.0")
+    B = G.Bounder(SPL, G.default_params(), batch=8)
+    opt = B.bab()
+    print("g05_60.0 nodes", B.ctx.nodes, "root", B.ctx.root_bound)
+    assert opt == 536.0
+    x = np.array(B.ctx.best_x[:59], dtype=float)
+    assert x @ SPL[:59, :59] @ x == 536.0
