@@ -66,6 +66,8 @@ typedef struct {
     int cg_maxit;          /* default 1000 */
     int jacobi_sweeps;     /* max Jacobi sweeps per eigen-solve, default 40 */
     int check_every;       /* ADMM iterations between host checks of the stop flags, default 16 */
+    int projection;        /* 0: FP64 eigen-decomposition (Jacobi); 1: mixed-precision sign
+                              iteration (FP32 MFMA quintic Newton-Schulz + FP64 finish) */
 } madam_opts;
 
 /* One ADMM_solver call (admm.c:49): inputs are read, outputs written.
@@ -100,6 +102,14 @@ void madam_close(void);
 /* Last error message (static storage). */
 const char *madam_last_error(void);
 
+/* Self-test of the MFMA matrix paths used by the mixed-precision projection:
+ * C32 = A32 (alpha I + beta B32) with FP32 MFMA (np x np, np % 16 == 0) and
+ * C64 = A64 (alpha I + beta B64) with FP64 MFMA (n x n); A, B symmetric and
+ * commuting so that the product is symmetric (lower tiles are mirrored). */
+int madam_mfma_selftest(int np, const float *A32, const float *B32, float *C32, int n,
+                        const double *A64, const double *B64, double *C64, double alpha,
+                        double beta);
+
 /* Per-phase GPU timing (no reference counterpart; the reference only takes
  * MPI_Wtime stamps, admm.c:674).  Phases, in order: 0 prep (y, rhs),
  * 1 cg (BB^T+I solve), 2 w (duals + W), 3 eig (projection eigen-solve),
@@ -126,6 +136,11 @@ int madam_op_Bt(int n, double *X, int ntri, const madam_tri *tri, int npent,
 /* project_SDP (operators.c:521): X = W_+ (eigenvalues > 1e-8), Q = X - W,
  * Z = eigenvectors * sqrt(lambda) (n x M, ascending lambda), M. */
 int madam_project_sdp(int n, const double *W, double *X, double *Q, double *Z, int *M);
+
+/* project_SDP in the mixed-precision form: X = W_+ = (W + W sign(W)) / 2
+ * with sign(W) from the FP32-MFMA quintic sign iteration and one FP64
+ * correction step, Q = X - W, M = number of positive eigenvalues. */
+int madam_project_sdp_ns(int n, const double *W, double *X, double *Q, int *M);
 
 /* Certified largest eigenvalue in FP64 (the dsyevr call of admm.c:731-747):
  * returns an upper bound lam >= lambda_max(A) within |E|_F + 1e-12 |A|_F. */
@@ -185,7 +200,7 @@ typedef struct {
     double diff;                    /* basic SDP bound - cut bound at the root */
     unsigned long long seed;        /* random streams of SA and GW rounding */
     long long stream_ctr;           /* in/out: next stream id */
-    int projection;                 /* 0: FP64 eigen-decomposition projection */
+    int projection;                 /* as madam_opts.projection */
     int batch;                      /* nodes bounded together by madam_bab */
     long long nodes;                /* out: evaluated nodes */
     long long admm_iters, admm_calls, ipm_iters;

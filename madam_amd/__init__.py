@@ -48,7 +48,7 @@ class Hept(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("cg_rtol", C.c_double), ("cg_maxit", C.c_int),
-                ("jacobi_sweeps", C.c_int), ("check_every", C.c_int)]
+                ("jacobi_sweeps", C.c_int), ("check_every", C.c_int), ("projection", C.c_int)]
 
 
 class AdmmJob(C.Structure):
@@ -158,6 +158,18 @@ def project_SDP(W):
     return X, Q, Z[:, :M.value].copy(), M.value
 
 
+def project_SDP_ns(W):
+    """Mixed-precision project_SDP (sign iteration on MFMA): X_+, Q, M."""
+    W = _f64(W)
+    n = W.shape[0]
+    X = np.zeros((n, n), order="F")
+    Q = np.zeros((n, n), order="F")
+    M = C.c_int(0)
+    _check(lib().madam_project_sdp_ns(C.c_int(n), W.ctypes.data_as(_PD), X.ctypes.data_as(_PD),
+                                      Q.ctypes.data_as(_PD), C.byref(M)))
+    return X, Q, M.value
+
+
 def lambda_max(A):
     """Certified upper bound on lambda_max(A) (admm.c:731-747)."""
     A = _f64(A)
@@ -202,9 +214,11 @@ def _make_job(L, X, y, Q, tri, pent, hept, sigma, tol, itermax, scaleSigma, keep
     return j, dict(X=X, Q=Q, y=y, s=s, t=t, u=u, Z=Z, sigma=sg, m=m)
 
 
-def _opts(cg_rtol=1e-12, cg_maxit=1000, jacobi_sweeps=40, check_every=16):
+def _opts(cg_rtol=1e-12, cg_maxit=1000, jacobi_sweeps=40, check_every=16, projection=0):
+    """projection: 0 FP64 eigen (Jacobi), 1 mixed-precision sign iteration."""
     o = Opts()
     o.cg_rtol, o.cg_maxit, o.jacobi_sweeps, o.check_every = cg_rtol, cg_maxit, jacobi_sweeps, check_every
+    o.projection = projection
     return o
 
 
@@ -405,7 +419,7 @@ def gw_rounding(L, Z, num, seed=2020, stream=0):
 class Bounder:
     """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
 
-    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0):
+    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0):
         self.SPL = _f64(SPL)
         self.ctx = BabCtx()
         self.ctx.N = self.SPL.shape[0]
@@ -416,6 +430,7 @@ class Bounder:
         self.ctx.seed = seed
         self.ctx.batch = batch
         self.ctx.LB = LB
+        self.ctx.projection = projection
 
     def sdpbound(self, nodes, ranks=None):
         k = len(nodes)
@@ -439,3 +454,20 @@ class Bounder:
         opt = C.c_double(0)
         _check(lib().madam_bab(C.byref(self.ctx), C.c_longlong(max_nodes), C.byref(opt)))
         return opt.value
+
+
+def mfma_selftest(A32, B32, A64, B64, alpha, beta):
+    """madam_mfma_selftest: (C32, C64) products through the MFMA paths."""
+    A32 = np.asfortranarray(A32, dtype=np.float32)
+    B32 = np.asfortranarray(B32, dtype=np.float32)
+    A64 = _f64(A64)
+    B64 = _f64(B64)
+    np_, n = A32.shape[0], A64.shape[0]
+    C32 = np.zeros((np_, np_), np.float32, order="F")
+    C64 = np.zeros((n, n), order="F")
+    _check(lib().madam_mfma_selftest(C.c_int(np_), A32.ctypes.data_as(C.POINTER(C.c_float)),
+                                     B32.ctypes.data_as(C.POINTER(C.c_float)),
+                                     C32.ctypes.data_as(C.POINTER(C.c_float)), C.c_int(n),
+                                     A64.ctypes.data_as(_PD), B64.ctypes.data_as(_PD),
+                                     C64.ctypes.data_as(_PD), C.c_double(alpha), C.c_double(beta)))
+    return C32, C64

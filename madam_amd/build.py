@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libmadam_bound.so")
-SOURCES = ["madam_device.hip", "madam_sep.hip", "madam_ipm.hip", "madam_gw.hip",
+SOURCES = ["madam_device.hip", "madam_sep.hip", "madam_ipm.hip", "madam_gw.hip", "madam_ns.hip",
            "madam_host.hip", "madam_bab.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

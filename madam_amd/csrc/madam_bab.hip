@@ -387,6 +387,7 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
       return rc;
     Slot& sl = E.slots[r.slot];
     if ((rc = sl.size_for(r.n, cs[q])) || (rc = upload_cuts(sl, cs[q], st))) return rc;
+    if (c->projection == 1 && (rc = sl.size_ns(r.n))) return rc;
     if (cs[q].m > 0) {
       if ((rc = up(sl.s.p, cs[q].y0.data(), cs[q].m, st)) ||
           (rc = up(sl.t.p, cs[q].y0.data(), cs[q].m, st)))
@@ -412,6 +413,7 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   o.cg_maxit = 1000;
   o.jacobi_sweeps = 40;
   o.check_every = 16;
+  o.projection = c->projection;
   if ((rc = admm_loop(hv, o, nmax, mmax))) return rc;
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];

@@ -358,6 +358,7 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
     if (!values_only) nd.eig_warm = 1;
   }
   if (!values_only && form_xplus) {
+    // form_xplus == 2: only lambda_+ (evp) and M, no X_+
     // lambda_+ into the rotation arrays (3h >= n doubles, free after the sweeps)
     double* dpos = rc;
     __syncthreads();
@@ -371,7 +372,9 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
     }
     block_sums<1>(mv, red);
     if (tid == 0) nd.M = (int)mv[0];
-    if (V_LDS) {
+    if (form_xplus == 2) {
+      // eigen-pairs only
+    } else if (V_LDS) {
       // register-blocked X_+ (4x4 per thread) straight from LDS V
       const int nb4 = (n + 3) >> 2;
       const int nbl = nb4 * (nb4 + 1) / 2;

@@ -174,6 +174,14 @@ struct Slot {
   DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
   DBuf<double> cscale, t, s, u, rhs, r, p, q, w;
   DBuf<double> ip1, ip2, ip3, ipv, ipP;  // IPM scratch
+  DBuf<float> ns32;                      // sign-iteration scratch
+
+  int size_ns(int n) {
+    const size_t np = (size_t)((n + 15) & ~15);
+    int rc;
+    if ((rc = ns32.ensure(3 * np * np)) || (rc = ip1.ensure((size_t)n * n))) return rc;
+    return MADAM_OK;
+  }
   DBuf<double> gwval, gwbest;            // GW per-hyperplane values, best value
   DBuf<signed char> gwz, gwbz;           // GW cuts, best cut
   int n_ipm = 0;
@@ -218,6 +226,7 @@ struct Slot {
     for (DBuf<double>* b : {&ip1, &ip2, &ip3, &ipv, &ipP, &gwval, &gwbest}) b->release();
     gwz.release();
     gwbz.release();
+    ns32.release();
   }
 };
 
@@ -320,6 +329,7 @@ inline madam_opts default_opts() {
   o.cg_maxit = 1000;
   o.jacobi_sweeps = 40;
   o.check_every = 16;
+  o.projection = 0;
   return o;
 }
 
@@ -373,6 +383,7 @@ inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
   v.ip3 = sl.ip3.p;
   v.ipv = sl.ipv.p;
   v.ipP = sl.ipP.p;
+  v.ns32 = sl.ns32.p;
 }
 
 inline int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {

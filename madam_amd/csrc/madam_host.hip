@@ -52,8 +52,12 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
-      TIMED(PH_EIG, st,
-            madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1));
+      if (o.projection == 1) {
+        TIMED(PH_EIG, st, madam::launch_ns_project(E.views.p, k, 0, st));
+      } else {
+        TIMED(PH_EIG, st,
+              madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1));
+      }
       TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, st));
     }
     HIP_TRY(hipGetLastError());
@@ -63,6 +67,11 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     bool all = true;
     for (int b = 0; b < k; ++b) all = all && hv[b].done;
     if (all) break;
+  }
+  if (o.projection == 1) {
+    // eigen-pairs of the last W for the rounding heuristic's factor Z
+    TIMED(PH_EIG, st,
+          madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 1, E.lds_limit, st, 2));
   }
   TIMED(PH_SAFE, st, {
     madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
@@ -175,6 +184,7 @@ int madam_admm_batch(int k, madam_admm_job* jobs, const madam_opts* opts_in) {
     madam_admm_job& J = jobs[b];
     Slot& sl = E.slots[b];
     if ((rc = sl.size_for(J.n, cs[b]))) return rc;
+    if (o.projection == 1 && (rc = sl.size_ns(J.n))) return rc;
     const size_t nn = (size_t)J.n * J.n;
     if ((rc = up(sl.L.p, J.L, nn, st)) || (rc = up(sl.X.p, J.X, nn, st)) ||
         (rc = up(sl.Q.p, J.Q, nn, st)) || (rc = up(sl.y.p, J.y, (size_t)J.n, st)) ||
@@ -487,6 +497,56 @@ int madam_qap_sa(int n, const double* X, int k, int nruns, const int* types,
     for (int a = 0; a < k; ++a) perms[(size_t)r * k + a] = out[r].perm[a];
     if (draws) draws[r] = out[r].draws;
   }
+  return MADAM_OK;
+}
+
+
+int madam_mfma_selftest(int np, const float* A32, const float* B32, float* C32, int n,
+                        const double* A64, const double* B64, double* C64, double alpha,
+                        double beta) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (np <= 0 || np % 16 || n <= 0 || n > MADAM_NMAX || np > MADAM_NMAX) {
+    set_err("madam_mfma_selftest: bad sizes");
+    return MADAM_E_ARG;
+  }
+  const hipStream_t st = g_eng->stream;
+  DBuf<float> a32, b32, c32;
+  DBuf<double> a64, b64, c64;
+  const size_t p2 = (size_t)np * np, n2 = (size_t)n * n;
+  if ((rc = a32.ensure(p2)) || (rc = b32.ensure(p2)) || (rc = c32.ensure(p2)) ||
+      (rc = a64.ensure(n2)) || (rc = b64.ensure(n2)) || (rc = c64.ensure(n2)))
+    return rc;
+  if ((rc = up(a32.p, A32, p2, st)) || (rc = up(b32.p, B32, p2, st)) || (rc = up(a64.p, A64, n2, st)) ||
+      (rc = up(b64.p, B64, n2, st)))
+    return rc;
+  madam::launch_mfma_test(a32.p, b32.p, c32.p, np, a64.p, b64.p, c64.p, n, alpha, beta, st);
+  HIP_TRY(hipGetLastError());
+  if ((rc = down(C32, c32.p, p2, st)) || (rc = down(C64, c64.p, n2, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  a32.release(); b32.release(); c32.release(); a64.release(); b64.release(); c64.release();
+  return MADAM_OK;
+}
+
+
+int madam_project_sdp_ns(int n, const double* W, double* X, double* Q, int* M) {
+  CutSet cs;
+  int rc = one_node(n, 0, nullptr, 0, nullptr, 0, nullptr, cs);
+  if (rc) return rc;
+  Slot& sl = g_eng->slots[0];
+  if ((rc = sl.size_ns(n))) return rc;
+  bind_view(g_eng->hviews[0], sl, n, cs);
+  const hipStream_t st = g_eng->stream;
+  const size_t nn = (size_t)n * n;
+  if ((rc = up(g_eng->views.p, g_eng->hviews.data(), 1, st)) || (rc = up(sl.W.p, W, nn, st))) return rc;
+  TIMED(PH_EIG, st, madam::launch_ns_project(g_eng->views.p, 1, 1, st));
+  madam::launch_project(g_eng->views.p, 1, n, st);
+  HIP_TRY(hipGetLastError());
+  if ((rc = sync_views_down(1))) return rc;
+  if ((rc = down(X, sl.X.p, nn, st)) || (rc = down(Q, sl.Q.p, nn, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  g_eng->timer.flush();
+  if (M) *M = g_eng->hviews[0].M;
   return MADAM_OK;
 }
 

@@ -33,6 +33,7 @@ struct NodeView {
   double *ip1, *ip2, *ip3;             // n*n IPM scratch
   double *ipv;                         // 4n IPM vectors
   double *ipP;                         // n(n+1)/2 packed Cholesky factor (when not in LDS)
+  float *ns32;                         // 3 n_p^2 FP32 sign-iteration scratch (n_p = n rounded up to 16)
 
   const int *cptr;      // m+1
   const int *cent;      // cptr[m]: lo or ~lo
@@ -76,5 +77,9 @@ void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout
                   hipStream_t stream);
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t stream);
+void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t stream);
+void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, const double* A64,
+                      const double* B64, double* C64, int n, double alpha, double beta,
+                      hipStream_t stream);
 
 }  // namespace madam

@@ -1,0 +1,189 @@
+// madam_ns.hip -- mixed-precision PSD projection by a matrix sign
+// iteration on MFMA (the chosen form of project_SDP, operators.c:521-619).
+//
+// W_+ = (W + W sign(W)) / 2.  sign(W) is computed from S_0 = W / |W|_F
+// (spectrum inside [-1, 1]) by 12 steps S <- a S + b S^3 + c S^5 whose
+// coefficients are the minimax odd quintics for the interval the spectrum
+// occupies after the previous step, starting from [1e-6, 1]: the measured
+// smallest |lambda| / |W|_F of ADMM's W goes down to ~1e-6 on the torus
+// instances.  Each step is three symmetric products on FP32 MFMA
+// (v_mfma_f32_16x16x4f32, FP32 accumulate): T = S S, U = T (b I + c T),
+// S' = S (a I + U); all iterates are polynomials in W, so every product is
+// symmetric and only lower 16x16 tiles are computed and mirrored.  One FP64
+// correction step S <- S (3 I - S^2) / 2 on FP64 MFMA
+// (v_mfma_f64_16x16x4f64) restores S^2 = I to FP64 precision, and the
+// projection itself, W_+ = (W + W S) / 2, is an FP64 product.
+//
+// One workgroup per node; the FP32 iterates live in a per-node scratch
+// (3 n_p^2 floats, n_p = n rounded up to 16, zero padded), FP64 matrices in
+// the node's S1/S2.  Output: X_+ in S1 (full), M = round(trace((I+S)/2)).
+#include "madam_devutil.h"
+#include "madam_internal.h"
+
+namespace madam {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// minimax odd quintic schedule (see header comment)
+__constant__ float NS_COEF[12][3] = {
+    {6.5789436f, -12.24106284f, 7.66211266f}, {4.2570903f, -3.16017155f, 0.58647659f},
+    {4.25679561f, -3.15982678f, 0.58639591f}, {4.25610454f, -3.15951454f, 0.58640776f},
+    {4.25249924f, -3.15681172f, 0.58606352f}, {4.21653548f, -3.10188261f, 0.57502521f},
+    {4.17739833f, -3.10488706f, 0.58016048f}, {3.93924782f, -2.93408607f, 0.56002186f},
+    {3.23597078f, -2.42721147f, 0.50310531f}, {2.23586608f, -1.61014368f, 0.412479f},
+    {1.88551957f, -1.26163833f, 0.37616478f}, {1.87500592f, -1.25001147f, 0.37500556f}};
+constexpr int NS_STEPS = 12;
+
+// C = A (alpha I + beta B) for symmetric n_p x n_p A, B (column-major, ld
+// n_p, n_p % 16 == 0) whose product is symmetric; FP32 MFMA, lower tiles.
+// Lane layout of v_mfma_f32_16x16x4f32: A operand lane l -> (row l%16,
+// k l/16); B operand lane l -> (k l/16, col l%16); result lane l -> rows
+// 4(l/16)+v, col l%16.
+__device__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
+                             float* __restrict__ Cm, int np, float alpha, float beta) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nt = np >> 4;
+  const int ntile = nt * (nt + 1) / 2;
+  const int r = lane & 15, q = lane >> 4;
+  for (int t = wave; t < ntile; t += nw) {
+    int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    if (ti * (ti + 1) / 2 > t) --ti;
+    const int tj = t - ti * (ti + 1) / 2;
+    const int i0 = ti << 4, j0 = tj << 4;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < np; k0 += 4) {
+      const int k = k0 + q;
+      const float a = A[(i0 + r) + (size_t)k * np];
+      // B symmetric: B[k][j] = B[j + k np]
+      float b = beta * B[(j0 + r) + (size_t)k * np];
+      if (j0 + r == k) b += alpha;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+    // only the lower element of each (i,j)/(j,i) pair is stored, to both
+    // places: the iterates stay exactly symmetric, which the transposed
+    // operand reads rely on (an asymmetric part would grow step by step)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + 4 * q + v, j = j0 + r;
+      if (i >= j) {
+        Cm[i + (size_t)j * np] = acc[v];
+        Cm[j + (size_t)i * np] = acc[v];
+      }
+    }
+  }
+}
+
+// Same in FP64 with v_mfma_f64_16x16x4f64, on n x n matrices of any n
+// (tiles past n are masked).  op: C = A (alpha I + beta B).  Operand
+// layouts as in the FP32 case; the result register v of lane l holds row
+// (l/16) + 4v, column l%16 (measured on gfx950).
+__device__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+                             double* __restrict__ Cm, int n, double alpha, double beta) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nt = (n + 15) >> 4;
+  const int ntile = nt * (nt + 1) / 2;
+  const int r = lane & 15, q = lane >> 4;
+  for (int t = wave; t < ntile; t += nw) {
+    int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    if (ti * (ti + 1) / 2 > t) --ti;
+    const int tj = t - ti * (ti + 1) / 2;
+    const int i0 = ti << 4, j0 = tj << 4;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      const int k = k0 + q;
+      const int ia = i0 + r, jb = j0 + r;
+      const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
+      double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
+      if (jb == k && k < n) b += alpha;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + q + 4 * v, j = j0 + r;
+      if (i < n && j < n && i >= j) {
+        Cm[i + (size_t)j * n] = acc[v];
+        Cm[j + (size_t)i * n] = acc[v];
+      }
+    }
+  }
+}
+
+// Projection of nd.W (FP64) -> X_+ in nd.S1.  ns32: 3 n_p^2 floats.
+__global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore_done) {
+  __shared__ double red[64];
+  NodeView& nd = nodes[blockIdx.x];
+  if (!ignore_done && nd.done) return;
+  const int n = nd.n, np = (n + 15) & ~15, tid = threadIdx.x;
+  const size_t npp = (size_t)np * np;
+  float* Sb = nd.ns32;
+  float* Tb = Sb + npp;
+  float* Ub = Tb + npp;
+  const double* __restrict__ Wm = nd.W;
+  double fv[1] = {0.0};
+  for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
+  block_sums<1>(fv, red);
+  const double fro = sqrt(fv[0]);
+  const float inv = fro > 0.0 ? (float)(1.0 / fro) : 0.f;
+  for (size_t f = tid; f < npp; f += WG) {
+    const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
+    Sb[f] = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
+  }
+  __syncthreads();
+  float* S = Sb;
+  float* T = Tb;
+  float* U = Ub;
+  for (int s = 0; s < NS_STEPS; ++s) {
+    const float a = NS_COEF[s][0], b = NS_COEF[s][1], c = NS_COEF[s][2];
+    sym_gemm_f32(S, S, T, np, 0.f, 1.f);  // T = S^2
+    __syncthreads();
+    sym_gemm_f32(T, T, U, np, b, c);      // U = T (b I + c T)
+    __syncthreads();
+    sym_gemm_f32(S, U, T, np, a, 1.f);    // S' = S (a I + U), into T
+    __syncthreads();
+    float* tmp = S;
+    S = T;
+    T = tmp;
+  }
+  // FP64 correction S <- S (3 I - S^2) / 2 and W_+ = (W + W S) / 2
+  double* S64 = nd.S2;
+  double* T64 = nd.S1;
+  for (int f = tid; f < n * n; f += WG) {
+    const int j = f / n, i = f - j * n;
+    S64[f] = (double)S[i + (size_t)j * np];
+  }
+  __syncthreads();
+  sym_gemm_f64(S64, S64, T64, n, 0.0, 1.0);  // T = S^2
+  __syncthreads();
+  sym_gemm_f64(S64, T64, nd.ip1, n, 1.5, -0.5);  // S' = S (1.5 I - 0.5 S^2)
+  __syncthreads();
+  double tr[1] = {0.0};
+  for (int i = tid; i < n; i += WG) tr[0] += nd.ip1[(size_t)i * (n + 1)];
+  block_sums<1>(tr, red);
+  // X_+ = (W + W S) / 2 = W (0.5 I + 0.5 S)
+  sym_gemm_f64(Wm, nd.ip1, nd.S1, n, 0.5, 0.5);
+  if (tid == 0) nd.M = (int)lrint(0.5 * (n + tr[0]));
+}
+
+void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t st) {
+  k_ns_project<<<nb, 1024, 0, st>>>(nodes, ignore_done);
+}
+
+// Self-test of the MFMA operand/result layouts: C32 = A (alpha I + beta B)
+// in FP32 MFMA (np x np), C64 the same in FP64 MFMA (n x n).
+__global__ __launch_bounds__(1024) void k_mfma_test(const float* A32, const float* B32, float* C32,
+                                                   int np, const double* A64, const double* B64,
+                                                   double* C64, int n, double alpha, double beta) {
+  sym_gemm_f32(A32, B32, C32, np, (float)alpha, (float)beta);
+  sym_gemm_f64(A64, B64, C64, n, alpha, beta);
+}
+
+void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, const double* A64,
+                      const double* B64, double* C64, int n, double alpha, double beta,
+                      hipStream_t st) {
+  k_mfma_test<<<1, 1024, 0, st>>>(A32, B32, C32, np, A64, B64, C64, n, alpha, beta);
+}
+
+}  // namespace madam
