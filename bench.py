@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X bounding path.  Prints ONE JSON line.
+"""Benchmark of the MI355X SDP bounding path.  Prints ONE JSON line.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                [--no-cpu] [--cpu-seconds S]
+                [--no-cpu] [--no-plain] [--cpu-seconds S]
 
-A "step" is one pass of the hot path over one batch of synthetic input (the
-workload table below says what one step bounds).  The timed region covers
-exactly K steps; inputs are generated before it.  `value` is node bounds per
-second for the whole job; `roofline` prices the dominant kernel from its
-HIP-event time measured inside this run; `cpu_baseline` times the CPU FP64
-oracle (LAPACK ADMM) on a bounded sample of the same workload on this host.
+Workloads (a "step" bounds the nodes listed; inputs are built before the
+timed region, instance data is uploaded inside each bound call):
+  c2-root   config 2: root SDPbound of G(100, 0.3), integer weights
+            uniform on +-{1..10}, seed 2 (IPM + cutting-plane loop with
+            triangle, pentagonal and heptagonal cuts, params file values)
+  c3-batch  config 3: 64 nodes of the 15x15 toroidal grid (weights +-1,
+            seed 3) with 0..75 random fixings each (n 150..225), bounded
+            in one batch; triangle + pentagonal cuts (TriIneq 1500,
+            PentIneq 500), no heptagonal cuts, no diff give-up
+  c1-bab    config 1: full branch and bound of g05_60.0 (rudy), time to
+            the proven optimum
+The metric value is node bounds per second of the chosen form (batched,
+mixed-precision sign-iteration projection); the plain FP64 GPU form
+(Jacobi eigen-projection) is timed in the same run, and `cpu_baseline`
+times the CPU FP64 oracle (LAPACK ADMM) on a bounded sample of the same
+nodes, one core.  `roofline` prices the projection kernel from its
+HIP-event time in this run.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -28,127 +40,217 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "time-to-proven-optimum + node bounds/sec, chosen vs plain FP64 GPU at 1e-6"
-FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
-PHASES = ["prep", "cg", "w", "eig", "post", "safe", "ipm", "trisep", "sa", "gw"]
+PEAK = {"fp32-mfma": 157.3, "fp64": 78.6}
+PHASES = ["prep", "cg", "w", "eig", "post", "safe", "ipm", "trisep", "sa", "gw", "zfac"]
 
 
-def root_L_config2():
-    from madam_amd import instances as I
-    nv, ei, ej, w = I.config2(seed=2)
-    SPL = I.objective_matrix(nv, ei, ej, w)
-    return SPL, np.asfortranarray(0.25 * SPL)   # createSubproblem with nothing fixed
+def proj_flops(n, form):
+    """Algorithmic flops of one projection.
+    chosen: 36 FP32 products + 3 FP64 products of order n (2 n^3 each);
+    plain: nominal LAPACK count (10/3) n^3 + n^2 M with M ~ n/2."""
+    if form == "chosen":
+        return 2.0 * n ** 3 * 39
+    return (10.0 / 3.0) * n ** 3 + n ** 3 / 2.0
 
 
-def workload_admm_basic(args):
-    """One step: the safe ADMM bound of the basic SDP (no cuts) at the root
-    of the config-2 instance (G(100, 0.3), integer weights +-1..10, seed 2),
-    cold start X = I, Q = 0, tolerance 1e-3 (params minTol)."""
+class Timer:
+    def __init__(self, lib):
+        self.lib = lib
+
+    def __enter__(self):
+        self.lib.madam_profile(1)
+        self.read()
+        return self
+
+    def read(self):
+        ms = np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), dtype=np.int64)
+        self.lib.madam_profile_read(len(PHASES), ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                    cnt.ctypes.data_as(C.POINTER(C.c_long)), 1)
+        return ms, cnt
+
+    def __exit__(self, *a):
+        self.ms, self.cnt = self.read()
+        self.lib.madam_profile(0)
+
+
+def wl_c2(args):
     import madam_amd as G
-    SPL, L = root_L_config2()
-    n = L.shape[0]
-    case = dict(L=L, X=np.eye(n, order="F"), y=np.zeros(n), Q=np.zeros((n, n), order="F"),
-                sigma=1.0, tol=1e-3)
+    from madam_amd import instances as I
+    SPL = I.objective_matrix(*I.config2(seed=2))
+    n = SPL.shape[0]
 
-    def step():
-        return G.ADMM_solver(**case)
+    def step(form):
+        B = G.Bounder(SPL, G.default_params(), projection=1 if form == "chosen" else 0)
+        b = B.sdpbound([B.node()], ranks=[0])[0]
+        return {"bound": float(b), "admm_iters": int(B.ctx.admm_iters), "nodes": 1,
+                "n3_iters": float(B.ctx.admm_iters) * n ** 3}
 
-    def nominal_flops(res):
-        # nominal LAPACK count of one projection: (10/3) n^3 + n^2 M
-        return (10.0 / 3.0) * n ** 3 + n * n * res.M
-
-    def cpu_once():
+    def cpu(seconds):
+        import ctypes as CC
         from oracle import oracle_py as O
-        sol = O.Solver(SPL, O.default_params())
-        sol.load_subproblem()
-        sol.mat("X")[:, :] = np.eye(n)
-        sol.mat("Q")[:, :] = 0.0
-        sol.set_cuts()
-        return sol.admm(1.0, 1e-3)
+        t0 = time.perf_counter()
+        reps, bound = 0, None
+        while reps == 0 or time.perf_counter() - t0 < seconds:
+            CC.CDLL(None).srand(2020)
+            s = O.Solver(SPL, O.default_params())
+            bound, _ = s.root_bound()
+            reps += 1
+        dt = time.perf_counter() - t0
+        return reps / dt, f"{reps} oracle root SDPbound evaluations of the same node in {dt:.1f} s", bound
 
-    cfg = {"workload": "admm-basic-sdp root bound, config 2: G(100,0.3) w in +-{1..10} seed 2, "
-                       "n=100, no cuts, tol 1e-3, cold start",
-           "n": n, "nodes_per_step": 1, "form": "plain-fp64"}
-    return cfg, step, nominal_flops, cpu_once
+    cfg = {"workload": "config 2 root SDPbound: G(100,0.3), w uniform on +-{1..10}, seed 2, "
+                       "n=100, tri+pent+hept cuts, params-file values",
+           "n": n, "nodes_per_step": 1}
+    return cfg, step, cpu
 
 
-WORKLOADS = {"admm-basic-c2": workload_admm_basic}
+def wl_c3(args):
+    import madam_amd as G
+    from madam_amd import instances as I
+    SPL = I.objective_matrix(*I.torus2d(15, 15, seed=3))
+    N = SPL.shape[0]
+    fix = I.random_fixings(N - 1, 64, seed=33, kmin=0, kmax=75)
+    params = dict(include_Hepta=0, TriIneq=1500, adjust_TriIneq=0, PentIneq=500, use_diff=0)
+
+    def step(form):
+        B = G.Bounder(SPL, G.default_params(**params), projection=1 if form == "chosen" else 0)
+        nodes = [B.node(xf, sx) for xf, sx in fix]
+        bs = B.sdpbound(nodes, ranks=[1] * len(nodes))
+        n3 = sum((N - int(xf.sum())) ** 3 for xf, _ in fix) / len(fix)
+        return {"bound": float(np.mean(bs)), "bounds": bs, "admm_iters": int(B.ctx.admm_iters),
+                "nodes": len(nodes), "n3_iters": float(B.ctx.admm_iters) * n3}
+
+    def cpu(seconds):
+        import ctypes as CC
+        from oracle import oracle_py as O
+        p = O.default_params(**params)
+        t0 = time.perf_counter()
+        done = 0
+        for xf, sx in fix:
+            CC.CDLL(None).srand(2020)
+            s = O.Solver(SPL, p)
+            node = s.load_subproblem(xf, sx)
+            s.evaluate(node, 1)
+            done += 1
+            if time.perf_counter() - t0 >= seconds:
+                break
+        dt = time.perf_counter() - t0
+        return done / dt, f"{done} of the 64 nodes bounded by the oracle in {dt:.1f} s", None
+
+    cfg = {"workload": "config 3 batch: 15x15 torus, w +-1 seed 3, 64 nodes with 0..75 random "
+                       "fixings (n 150..225) in one batch, tri+pent cuts (TriIneq 1500, "
+                       "PentIneq 500), no hepta, no diff give-up",
+           "n": "150..225", "nodes_per_step": 64}
+    return cfg, step, cpu
+
+
+def wl_c1(args):
+    import madam_amd as G
+    from madam_amd import instances as I
+    SPL = I.objective_matrix(*I.read_rudy(os.path.join(ROOT, "tests", "golden", "rudy", "g05_60.0")))
+
+    def step(form):
+        B = G.Bounder(SPL, G.default_params(), projection=1 if form == "chosen" else 0, batch=16)
+        opt = B.bab()
+        return {"bound": float(opt), "admm_iters": int(B.ctx.admm_iters), "nodes": int(B.ctx.nodes),
+                "n3_iters": float(B.ctx.admm_iters) * 60 ** 3, "root_bound": float(B.ctx.root_bound)}
+
+    def cpu(seconds):
+        import ctypes as CC
+        from oracle import oracle_py as O
+        CC.CDLL(None).srand(2020)
+        t0 = time.perf_counter()
+        s = O.Solver(SPL, O.default_params())
+        opt = s.bab()
+        dt = time.perf_counter() - t0
+        return s.s.nodes / dt, f"oracle B&B of g05_60.0: optimum {opt:.0f}, {s.s.nodes} nodes in {dt:.2f} s", opt
+
+    cfg = {"workload": "config 1: full B&B of g05_60.0 (G(60,0.5), w=1), params-file values",
+           "n": 60, "nodes_per_step": "B&B"}
+    return cfg, step, cpu
+
+
+WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="admm-basic-c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2-root", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the bounding path "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the bounding path "
               "shards by B&B node across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
 
     import madam_amd as G
     G.open_device(0)
-    cfg, step, nominal_flops, cpu_once = WORKLOADS[args.workload](args)
+    lib = G.lib()
+    cfg, step, cpu = WORKLOADS[args.workload](args)
 
     for _ in range(args.warmup):
-        step()
+        step("chosen")
     t0 = time.perf_counter()
-    res = None
+    nodes = 0
     for _ in range(args.steps):
-        res = step()           # each call returns after the device finished
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    nodes = cfg["nodes_per_step"]
-    value = nodes * args.steps / (t1 - t0)
+        r = step("chosen")
+        nodes += r["nodes"]
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = nodes / dt
 
-    # dominant-kernel timing from HIP events on the library's stream
-    ms = np.zeros(len(PHASES))
-    cnt = np.zeros(len(PHASES), dtype=np.int64)
-    import ctypes as C
-    lib = G.lib()
-    lib.madam_profile(1)
-    lib.madam_profile_read(len(PHASES), ms.ctypes.data_as(C.POINTER(C.c_double)),
-                           cnt.ctypes.data_as(C.POINTER(C.c_long)), 1)
-    pres = step()
-    lib.madam_profile_read(len(PHASES), ms.ctypes.data_as(C.POINTER(C.c_double)),
-                           cnt.ctypes.data_as(C.POINTER(C.c_long)), 1)
-    lib.madam_profile(0)
-    dom = int(np.argmax(ms))
+    with Timer(lib) as tm:
+        rp = step("chosen")
     eig = PHASES.index("eig")
-    launches = max(1, int(cnt[eig]))
-    avg_ms = ms[eig] / launches
-    flops_per_launch = nominal_flops(pres) * nodes
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    roofline = {"bound": "fp64", "kernel": "k_jacobi (projection eigen-solve)",
-                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "avg_launch_ms": avg_ms, "launches": launches,
-                "flops_per_launch": flops_per_launch,
-                "phase_ms": {p: round(float(v), 4) for p, v in zip(PHASES, ms) if cnt[PHASES.index(p)]},
-                "dominant_phase": PHASES[dom]}
+    eig_ms = float(tm.ms[eig])
+    flops = 2.0 * 39 * rp["n3_iters"]
+    achieved = flops / (eig_ms * 1e-3) / 1e12 if eig_ms > 0 else 0.0
+    roofline = {"bound": "mfma", "kernel": "k_ns_project (sign-iteration projection, FP32 MFMA "
+                "+ FP64 finish)", "achieved": achieved, "peak": PEAK["fp32-mfma"],
+                "unit": "TFLOP/s", "frac": achieved / PEAK["fp32-mfma"], "traffic": None,
+                "launches": int(tm.cnt[eig]), "avg_launch_ms": eig_ms / max(1, int(tm.cnt[eig])),
+                "flops_per_node_projection": "78 n^3 (36 FP32 + 3 FP64 products of order n)",
+                "projections": rp["admm_iters"],
+                "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
 
-    cpu = None
+    plain = None
+    if not args.no_plain:
+        t1 = time.perf_counter()
+        pr = step("plain")
+        tp = time.perf_counter() - t1
+        plain = {"value": pr["nodes"] / tp, "unit": "node bounds/s", "s_per_step": tp,
+                 "bound": pr["bound"], "admm_iters": pr["admm_iters"],
+                 "chosen_over_plain": value / (pr["nodes"] / tp),
+                 "rel_bound_diff_chosen_vs_plain": abs(r["bound"] - pr["bound"]) / abs(pr["bound"])}
+        if args.workload == "c3-batch":
+            diff = np.abs(r["bounds"] - pr["bounds"]) / np.abs(pr["bounds"])
+            plain["rel_bound_diff_max"] = float(diff.max())
+
+    cpub = None
     if not args.no_cpu:
-        t0 = time.perf_counter()
-        reps = 0
-        while time.perf_counter() - t0 < args.cpu_seconds or reps == 0:
-            cres = cpu_once()
-            reps += 1
-        tc = time.perf_counter() - t0
-        cpu = {"value": nodes * reps / tc / nodes * 1.0 if nodes == 1 else reps / tc,
-               "unit": "node bounds/s", "cores": 1, "kind": "port",
-               "sample": f"{reps} oracle ADMM bounds of the same node in {tc:.1f} s, 1 thread "
-                         f"(MKL_NUM_THREADS=1); oracle f={cres[0]:.6f} nbit={cres[2]}",
-               "gpu_f": res.f, "rel_err_f": abs(res.f - cres[0]) / abs(cres[0])}
+        cval, sample, cbound = cpu(args.cpu_seconds)
+        cpub = {"value": cval, "unit": "node bounds/s", "cores": 1, "kind": "port",
+                "sample": sample + " (1 thread, MKL_NUM_THREADS=1)"}
+        if cbound is not None:
+            cpub["bound"] = float(cbound)
+            cpub["rel_bound_diff_chosen_vs_oracle"] = abs(r["bound"] - cbound) / abs(cbound)
 
     line = {"metric": METRIC, "value": value, "unit": "node bounds/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic", "config": cfg, "roofline": roofline, "cpu_baseline": cpu,
-            "nbit": res.nbit, "f": res.f}
-    print(json.dumps(line))
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32-mfma+f64", "data": "synthetic", "config": cfg, "roofline": roofline,
+            "plain_fp64_gpu": plain, "cpu_baseline": cpub,
+            "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"]}}
+    if args.workload == "c1-bab":
+        line["time_to_optimum_s"] = ms_per_step / 1e3
+        line["optimum"] = r["bound"]
+    print(json.dumps(line, default=float))
 
 
 if __name__ == "__main__":

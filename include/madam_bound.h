@@ -114,9 +114,10 @@ int madam_mfma_selftest(int np, const float *A32, const float *B32, float *C32, 
  * MPI_Wtime stamps, admm.c:674).  Phases, in order: 0 prep (y, rhs),
  * 1 cg (BB^T+I solve), 2 w (duals + W), 3 eig (projection eigen-solve),
  * 4 post (X_+, residuals, stop test), 5 safe (safe bound), 6 ipm,
- * 7 trisep, 8 sa, 9 gw.  ms[p] = summed event time of all launches of
+ * 7 trisep, 8 sa, 9 gw, 10 zfac (eigen-pairs for the rounding heuristic in
+ * the sign-iteration form).  ms[p] = summed event time of all launches of
  * phase p, count[p] = number of timed launch groups. */
-#define MADAM_NPHASE 10
+#define MADAM_NPHASE 11
 int madam_profile(int on);
 int madam_profile_read(int nphase, double *ms, long *count, int reset);
 

@@ -232,7 +232,7 @@ struct Slot {
 
 // Optional per-phase GPU timing with HIP events on the engine's stream.
 enum Phase { PH_PREP, PH_CG, PH_W, PH_EIG, PH_POST, PH_SAFE, PH_IPM, PH_TRISEP, PH_SA, PH_GW,
-             PH_COUNT };
+             PH_ZFAC, PH_COUNT };
 
 struct PhaseTimer {
   bool on = false;

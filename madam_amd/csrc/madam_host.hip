@@ -70,7 +70,7 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
   }
   if (o.projection == 1) {
     // eigen-pairs of the last W for the rounding heuristic's factor Z
-    TIMED(PH_EIG, st,
+    TIMED(PH_ZFAC, st,
           madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 1, E.lds_limit, st, 2));
   }
   TIMED(PH_SAFE, st, {
