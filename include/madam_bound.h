@@ -143,6 +143,12 @@ int madam_project_sdp(int n, const double *W, double *X, double *Q, double *Z, i
  * correction step, Q = X - W, M = number of positive eigenvalues. */
 int madam_project_sdp_ns(int n, const double *W, double *X, double *Q, int *M);
 
+/* Coefficients (a_k, b_k, c_k) of the sign iteration's steps
+ * x -> a x + b x^3 + c x^5 used by the mixed-precision projection (no
+ * reference counterpart): writes min(cap, steps) triples to coef and
+ * returns the number of steps.  Runs on the host, no device needed. */
+int madam_ns_schedule(int cap, float *coef);
+
 /* Certified largest eigenvalue in FP64 (the dsyevr call of admm.c:731-747):
  * returns an upper bound lam >= lambda_max(A) within |E|_F + 1e-12 |A|_F. */
 int madam_lambda_max(int n, const double *A, double *lam);
@@ -208,6 +214,9 @@ typedef struct {
     long long open_nodes;           /* out: nodes left when max_nodes stopped the search */
     double root_bound;
     double seconds_ipm, seconds_admm, seconds_sep, seconds_heur;  /* host wall time */
+    double cg_rtol;                 /* PCG relative residual of the cut solve (<= 0: 1e-12) */
+    long long cg_iters;             /* out: PCG iterations */
+    long long ns_fallbacks;         /* out: sign-iteration projections redone by FP64 Jacobi */
 } madam_bab_ctx;
 
 /* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n

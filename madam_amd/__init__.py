@@ -170,6 +170,16 @@ def project_SDP_ns(W):
     return X, Q, M.value
 
 
+def ns_schedule():
+    """madam_ns_schedule: (steps, 3) float32 coefficients of the sign iteration."""
+    k = lib().madam_ns_schedule(0, None)
+    if k <= 0:
+        raise MadamError(lib().madam_last_error().decode())
+    out = np.zeros(3 * k, np.float32)
+    lib().madam_ns_schedule(C.c_int(k), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out.reshape(k, 3)
+
+
 def lambda_max(A):
     """Certified upper bound on lambda_max(A) (admm.c:731-747)."""
     A = _f64(A)
@@ -349,7 +359,8 @@ class BabCtx(C.Structure):
         ("nodes", C.c_longlong), ("admm_iters", C.c_longlong), ("admm_calls", C.c_longlong),
         ("ipm_iters", C.c_longlong), ("open_nodes", C.c_longlong), ("root_bound", C.c_double),
         ("seconds_ipm", C.c_double), ("seconds_admm", C.c_double), ("seconds_sep", C.c_double),
-        ("seconds_heur", C.c_double),
+        ("seconds_heur", C.c_double), ("cg_rtol", C.c_double), ("cg_iters", C.c_longlong),
+        ("ns_fallbacks", C.c_longlong),
     ]
 
 
@@ -419,7 +430,7 @@ def gw_rounding(L, Z, num, seed=2020, stream=0):
 class Bounder:
     """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
 
-    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0):
+    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0, cg_rtol=0.0):
         self.SPL = _f64(SPL)
         self.ctx = BabCtx()
         self.ctx.N = self.SPL.shape[0]
@@ -431,6 +442,7 @@ class Bounder:
         self.ctx.batch = batch
         self.ctx.LB = LB
         self.ctx.projection = projection
+        self.ctx.cg_rtol = cg_rtol
 
     def sdpbound(self, nodes, ranks=None):
         k = len(nodes)

@@ -409,7 +409,7 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     mmax = std::max(mmax, cs[q].m);
   }
   madam_opts o;
-  o.cg_rtol = 1e-12;
+  o.cg_rtol = c->cg_rtol > 0 ? c->cg_rtol : 1e-12;
   o.cg_maxit = 1000;
   o.jacobi_sweeps = 40;
   o.check_every = 16;
@@ -422,6 +422,8 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     r.nbit = hv[q].nbit;
     r.f = hv[q].f;
     c->admm_iters += hv[q].nbit;
+    c->cg_iters += hv[q].cg_total;
+    c->ns_fallbacks += hv[q].ns_fallbacks;
     c->admm_calls += 1;
     r.s.resize(cs[q].m);
     if (cs[q].m > 0 && (rc = down(r.s.data(), E.slots[r.slot].s.p, cs[q].m, st))) return rc;

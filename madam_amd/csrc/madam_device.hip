@@ -195,6 +195,8 @@ __global__ __launch_bounds__(WGS) void k_build_w(NodeView* nodes, int use_s) {
 // X_+ = V diag(lambda_+) V^T into S1 (full), evp and M, where lambda_+ keeps
 // eigenvalues > 1e-8 (the dsyevr range (1e-8, VU] of operators.c:536-579;
 // VU = min(|W|_inf, |W|_F) >= lambda_max, so only the lower end matters).
+// form_xplus == 3 is form 1 restricted to the nodes whose sign-iteration
+// projection failed its guard (nd.ns_fail); the other nodes return at once.
 
 struct PackedOut {
   double* A;
@@ -220,6 +222,11 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (!ignore_done && nd.done) return;
+  if (form_xplus == 3) {
+    // fallback behind the sign-iteration projection: only its failed nodes
+    if (!nd.ns_fail) return;
+    form_xplus = 1;
+  }
   const int n = nd.n;
   if (n <= 0) return;
   const int n2 = n + (n & 1), h = n2 >> 1;

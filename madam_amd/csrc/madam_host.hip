@@ -7,6 +7,7 @@
 // per-node stop flags on the host every `check_every` iterations; nodes that
 // stopped skip the remaining launches inside the kernels.
 #include "madam_engine.h"
+#include "madam_ns_coef.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -53,7 +54,10 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
       TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
-        TIMED(PH_EIG, st, madam::launch_ns_project(E.views.p, k, 0, st));
+        TIMED(PH_EIG, st, {
+          madam::launch_ns_project(E.views.p, k, 0, st);
+          madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 3);
+        });
       } else {
         TIMED(PH_EIG, st,
               madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1));
@@ -529,6 +533,17 @@ int madam_mfma_selftest(int np, const float* A32, const float* B32, float* C32, 
 }
 
 
+int madam_ns_schedule(int cap, float* coef) {
+  static const float tab[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
+  if (cap < 0 || (cap > 0 && !coef)) {
+    set_err("madam_ns_schedule: bad arguments");
+    return -MADAM_E_ARG;
+  }
+  for (int k = 0; k < MADAM_NS_STEPS && k < cap; ++k)
+    for (int j = 0; j < 3; ++j) coef[3 * k + j] = tab[k][j];
+  return MADAM_NS_STEPS;
+}
+
 int madam_project_sdp_ns(int n, const double* W, double* X, double* Q, int* M) {
   CutSet cs;
   int rc = one_node(n, 0, nullptr, 0, nullptr, 0, nullptr, cs);
@@ -539,7 +554,10 @@ int madam_project_sdp_ns(int n, const double* W, double* X, double* Q, int* M) {
   const hipStream_t st = g_eng->stream;
   const size_t nn = (size_t)n * n;
   if ((rc = up(g_eng->views.p, g_eng->hviews.data(), 1, st)) || (rc = up(sl.W.p, W, nn, st))) return rc;
-  TIMED(PH_EIG, st, madam::launch_ns_project(g_eng->views.p, 1, 1, st));
+  TIMED(PH_EIG, st, {
+    madam::launch_ns_project(g_eng->views.p, 1, 1, st);
+    madam::launch_jacobi(g_eng->views.p, 1, n, 0, 40, 1, g_eng->lds_limit, st, 3);
+  });
   madam::launch_project(g_eng->views.p, 1, n, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;

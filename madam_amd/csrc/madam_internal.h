@@ -58,6 +58,8 @@ struct NodeView {
   int eig_warm;   // V holds a usable basis from the previous projection
   int ipm_iters;  // IPM iterations of the last k_ipm
   int ipm_status; // 0 ok, else the reference's failure point (1..4)
+  int ns_fail;    // the latest sign-iteration projection failed its guard
+  int ns_fallbacks;  // projections redone by FP64 Jacobi after a failed guard
   int pad2;
 };
 

@@ -2,11 +2,13 @@
 // iteration on MFMA (the chosen form of project_SDP, operators.c:521-619).
 //
 // W_+ = (W + W sign(W)) / 2.  sign(W) is computed from S_0 = W / |W|_F
-// (spectrum inside [-1, 1]) by 12 steps S <- a S + b S^3 + c S^5 whose
-// coefficients are the minimax odd quintics for the interval the spectrum
-// occupies after the previous step, starting from [1e-6, 1]: the measured
-// smallest |lambda| / |W|_F of ADMM's W goes down to ~1e-6 on the torus
-// instances.  Each step is three symmetric products on FP32 MFMA
+// (spectrum inside [-1, 1]) by 13 steps S <- a S + b S^3 + c S^5 whose
+// coefficients (madam_ns_coef.h) drive every |x| >= 1e-6 to 1 without ever
+// squashing a large value towards 0 (which would let FP32 rounding rotate
+// its eigenvector); the measured smallest |lambda| / |W|_F of ADMM's W goes
+// down to ~1e-6 on the torus instances; eigenvalues below that are not
+// driven to +-1, which changes W_+ by at most |lambda| / 2 in their
+// eigen-directions.  Each step is three symmetric products on FP32 MFMA
 // (v_mfma_f32_16x16x4f32, FP32 accumulate): T = S S, U = T (b I + c T),
 // S' = S (a I + U); all iterates are polynomials in W, so every product is
 // symmetric and only lower 16x16 tiles are computed and mirrored.  One FP64
@@ -17,23 +19,21 @@
 // One workgroup per node; the FP32 iterates live in a per-node scratch
 // (3 n_p^2 floats, n_p = n rounded up to 16, zero padded), FP64 matrices in
 // the node's S1/S2.  Output: X_+ in S1 (full), M = round(trace((I+S)/2)).
+// Guard: an FP32 iterate with an entry above 1.5 in magnitude or not finite
+// (a sign matrix has all entries in [-1, 1]) sets nd.ns_fail and leaves S1
+// for the FP64 Jacobi projection that the host launches behind this kernel
+// (k_jacobi with form 3 runs only on such nodes).
 #include "madam_devutil.h"
 #include "madam_internal.h"
+#include "madam_ns_coef.h"
 
 namespace madam {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// minimax odd quintic schedule (see header comment)
-__constant__ float NS_COEF[12][3] = {
-    {6.5789436f, -12.24106284f, 7.66211266f}, {4.2570903f, -3.16017155f, 0.58647659f},
-    {4.25679561f, -3.15982678f, 0.58639591f}, {4.25610454f, -3.15951454f, 0.58640776f},
-    {4.25249924f, -3.15681172f, 0.58606352f}, {4.21653548f, -3.10188261f, 0.57502521f},
-    {4.17739833f, -3.10488706f, 0.58016048f}, {3.93924782f, -2.93408607f, 0.56002186f},
-    {3.23597078f, -2.42721147f, 0.50310531f}, {2.23586608f, -1.61014368f, 0.412479f},
-    {1.88551957f, -1.26163833f, 0.37616478f}, {1.87500592f, -1.25001147f, 0.37500556f}};
-constexpr int NS_STEPS = 12;
+__constant__ float NS_COEF[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
+constexpr int NS_STEPS = MADAM_NS_STEPS;
 
 // C = A (alpha I + beta B) for symmetric n_p x n_p A, B (column-major, ld
 // n_p, n_p % 16 == 0) whose product is symmetric; FP32 MFMA, lower tiles.
@@ -150,11 +150,24 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   // FP64 correction S <- S (3 I - S^2) / 2 and W_+ = (W + W S) / 2
   double* S64 = nd.S2;
   double* T64 = nd.S1;
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  __syncthreads();
   for (int f = tid; f < n * n; f += WG) {
     const int j = f / n, i = f - j * n;
-    S64[f] = (double)S[i + (size_t)j * np];
+    const float v = S[i + (size_t)j * np];
+    if (!(fabsf(v) <= 1.5f)) bad = 1;
+    S64[f] = (double)v;
   }
   __syncthreads();
+  if (bad) {
+    if (tid == 0) {
+      nd.ns_fail = 1;
+      nd.ns_fallbacks += 1;
+    }
+    return;
+  }
+  if (tid == 0) nd.ns_fail = 0;
   sym_gemm_f64(S64, S64, T64, n, 0.0, 1.0);  // T = S^2
   __syncthreads();
   sym_gemm_f64(S64, T64, nd.ip1, n, 1.5, -0.5);  // S' = S (1.5 I - 0.5 S^2)

@@ -83,3 +83,23 @@ def test_sdpbound_root_ns_matches_oracle():
     print("ns root", bg, "oracle", bo)
     assert bg >= 536.0
     assert abs(bg - bo) <= 1e-4 * abs(bo)
+
+
+@pytest.mark.parametrize("cg_rtol", [1e-10, 1e-6])
+def test_sdpbound_root_ns_stable_across_cut_solves(cg_rtol):
+    """Root SDPbound of config 2 (n=100, tri+pent+hept cuts): with a sign
+    schedule that lacks the stability margin, these cut-solve tolerances
+    steered the ADMM into a W whose sign iteration diverged (NaN bound).
+    The bound must be finite, valid against the plain FP64 form within the
+    spread of the cutting-plane loop, and need no Jacobi fallback."""
+    SPL = I.objective_matrix(*I.config2(seed=2))
+    B = G.Bounder(SPL, G.default_params(), projection=1, cg_rtol=cg_rtol)
+    bg = B.sdpbound([B.node()], ranks=[0])[0]
+    P = G.Bounder(SPL, G.default_params(), projection=0)
+    bp = P.sdpbound([P.node()], ranks=[0])[0]
+    print("cg_rtol", cg_rtol, "ns", bg, "plain", bp, "admm", B.ctx.admm_iters,
+          "fallbacks", B.ctx.ns_fallbacks)
+    assert np.isfinite(bg)
+    assert abs(bg - bp) <= 2e-3 * abs(bp)
+    assert B.ctx.ns_fallbacks == 0
+    assert B.ctx.admm_iters < 2000
