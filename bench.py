@@ -2,25 +2,30 @@
 """Benchmark of the MI355X SDP bounding path.  Prints ONE JSON line.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                [--no-cpu] [--no-plain] [--cpu-seconds S]
+                [--no-cpu] [--no-plain] [--cpu-seconds S] [--plain-nodes P]
 
 Workloads (a "step" bounds the nodes listed; inputs are built before the
 timed region, instance data is uploaded inside each bound call):
-  c2-root   config 2: root SDPbound of G(100, 0.3), integer weights
-            uniform on +-{1..10}, seed 2 (IPM + cutting-plane loop with
-            triangle, pentagonal and heptagonal cuts, params file values)
-  c3-batch  config 3: 64 nodes of the 15x15 toroidal grid (weights +-1,
-            seed 3) with 0..75 random fixings each (n 150..225), bounded
-            in one batch; triangle + pentagonal cuts (TriIneq 1500,
-            PentIneq 500), no heptagonal cuts, no diff give-up
-  c1-bab    config 1: full branch and bound of g05_60.0 (rudy), time to
-            the proven optimum
-The metric value is node bounds per second of the chosen form (batched,
-mixed-precision sign-iteration projection); the plain FP64 GPU form
-(Jacobi eigen-projection) is timed in the same run, and `cpu_baseline`
-times the CPU FP64 oracle (LAPACK ADMM) on a bounded sample of the same
-nodes, one core.  `roofline` prices the projection kernel from its
-HIP-event time in this run.
+  c2-root   root SDPbound of G(100, 0.3), integer weights uniform on
+            +-{1..10}, seed 2 (IPM + cutting-plane loop with triangle,
+            pentagonal and heptagonal cuts, params file values)
+  c3-batch  64 nodes of the 15x15 toroidal grid (weights +-1, seed 3) with
+            0..75 random fixings each (n 150..225), bounded in one batch;
+            triangle + pentagonal cuts (TriIneq 1500, PentIneq 500), no
+            heptagonal cuts, no diff give-up
+  c1-bab    full branch and bound of g05_60.0 (rudy), time to the proven
+            optimum
+Forms:
+  chosen    all nodes of a step in one batch (lockstep kernels, one
+            workgroup per node), sign-iteration projection on FP32 MFMA
+            with FP64 finish, PCG cut solve to 1e-7 relative residual
+  plain     one node at a time, FP64 Jacobi eigen-projection, PCG cut solve
+            to 1e-12 (a direct solve in the reference); timed on all nodes
+            of a step, or on the first --plain-nodes of c3-batch
+The metric value is node bounds per second of the chosen form.
+`cpu_baseline` times the CPU FP64 oracle (LAPACK ADMM, dense Cholesky cut
+solve) on a bounded sample of the same nodes, one core.  `roofline` prices
+the projection kernel from its HIP-event time in a separate profiled step.
 """
 from __future__ import annotations
 
@@ -40,27 +45,22 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "time-to-proven-optimum + node bounds/sec, chosen vs plain FP64 GPU at 1e-6"
-PEAK = {"fp32-mfma": 157.3, "fp64": 78.6}
+PEAK_FP32_MFMA = 157.3  # TFLOP/s, dense
+PEAK_FP64_MFMA = 78.6
 PHASES = ["prep", "cg", "w", "eig", "post", "safe", "ipm", "trisep", "sa", "gw", "zfac"]
+CHOSEN = dict(projection=1, cg_rtol=1e-7)
+PLAIN = dict(projection=0, cg_rtol=1e-12)
 
 
-def proj_flops(n, form):
-    """Algorithmic flops of one projection.
-    chosen: 39 FP32 products + 3 FP64 products of order n (2 n^3 each);
-    plain: nominal LAPACK count (10/3) n^3 + n^2 M with M ~ n/2."""
-    if form == "chosen":
-        return 2.0 * n ** 3 * 42
-    return (10.0 / 3.0) * n ** 3 + n ** 3 / 2.0
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 class Timer:
+    """Per-phase HIP-event times of the library (madam_profile)."""
+
     def __init__(self, lib):
         self.lib = lib
-
-    def __enter__(self):
-        self.lib.madam_profile(1)
-        self.read()
-        return self
 
     def read(self):
         ms = np.zeros(len(PHASES))
@@ -68,6 +68,11 @@ class Timer:
         self.lib.madam_profile_read(len(PHASES), ms.ctypes.data_as(C.POINTER(C.c_double)),
                                     cnt.ctypes.data_as(C.POINTER(C.c_long)), 1)
         return ms, cnt
+
+    def __enter__(self):
+        self.lib.madam_profile(1)
+        self.read()
+        return self
 
     def __exit__(self, *a):
         self.ms, self.cnt = self.read()
@@ -80,27 +85,26 @@ def wl_c2(args):
     SPL = I.objective_matrix(*I.config2(seed=2))
     n = SPL.shape[0]
 
-    def step(form):
-        B = G.Bounder(SPL, G.default_params(), projection=1 if form == "chosen" else 0)
+    def step(form, sample=None):
+        B = G.Bounder(SPL, G.default_params(), **(CHOSEN if form == "chosen" else PLAIN))
         b = B.sdpbound([B.node()], ranks=[0])[0]
-        return {"bound": float(b), "admm_iters": int(B.ctx.admm_iters), "nodes": 1,
-                "n3_iters": float(B.ctx.admm_iters) * n ** 3}
+        return {"bound": float(b), "bounds": np.array([b]), "admm_iters": int(B.ctx.admm_iters),
+                "nodes": 1, "n3_iters": float(B.ctx.admm_iters) * n ** 3}
 
     def cpu(seconds):
-        import ctypes as CC
         from oracle import oracle_py as O
         t0 = time.perf_counter()
         reps, bound = 0, None
         while reps == 0 or time.perf_counter() - t0 < seconds:
-            CC.CDLL(None).srand(2020)
+            C.CDLL(None).srand(2020)
             s = O.Solver(SPL, O.default_params())
             bound, _ = s.root_bound()
             reps += 1
         dt = time.perf_counter() - t0
         return reps / dt, f"{reps} oracle root SDPbound evaluations of the same node in {dt:.1f} s", bound
 
-    cfg = {"workload": "config 2 root SDPbound: G(100,0.3), w uniform on +-{1..10}, seed 2, "
-                       "n=100, tri+pent+hept cuts, params-file values",
+    cfg = {"workload": "root SDPbound of G(100,0.3), w uniform on +-{1..10}, seed 2, n=100, "
+                       "tri+pent+hept cuts, params-file values",
            "n": n, "nodes_per_step": 1}
     return cfg, step, cpu
 
@@ -112,35 +116,47 @@ def wl_c3(args):
     N = SPL.shape[0]
     fix = I.random_fixings(N - 1, 64, seed=33, kmin=0, kmax=75)
     params = dict(include_Hepta=0, TriIneq=1500, adjust_TriIneq=0, PentIneq=500, use_diff=0)
+    # Every node is bounded as its own subproblem (its own incumbent, from
+    # its own rounding heuristic), in both forms and in the oracle.
 
-    def step(form):
-        B = G.Bounder(SPL, G.default_params(**params), projection=1 if form == "chosen" else 0)
-        nodes = [B.node(xf, sx) for xf, sx in fix]
-        bs = B.sdpbound(nodes, ranks=[1] * len(nodes))
-        n3 = sum((N - int(xf.sum())) ** 3 for xf, _ in fix) / len(fix)
-        return {"bound": float(np.mean(bs)), "bounds": bs, "admm_iters": int(B.ctx.admm_iters),
-                "nodes": len(nodes), "n3_iters": float(B.ctx.admm_iters) * n3}
+    def step(form, sample=None):
+        sel = fix if sample is None else fix[:sample]
+        n3 = sum((N - int(xf.sum())) ** 3 for xf, _ in sel) / len(sel)
+        if form == "chosen":
+            B = G.Bounder(SPL, G.default_params(**params), own_incumbent=1, verbose=1, **CHOSEN)
+            bs = B.sdpbound([B.node(xf, sx) for xf, sx in sel], ranks=[1] * len(sel))
+            it = int(B.ctx.admm_iters)
+        else:
+            bs, it = [], 0
+            for q, (xf, sx) in enumerate(sel):
+                B = G.Bounder(SPL, G.default_params(**params), **PLAIN)
+                bs.append(B.sdpbound([B.node(xf, sx)], ranks=[1])[0])
+                it += int(B.ctx.admm_iters)
+                log(f"plain: node {q + 1} of {len(sel)} bounded")
+            bs = np.array(bs)
+        return {"bound": float(np.mean(bs)), "bounds": np.asarray(bs), "admm_iters": it,
+                "nodes": len(sel), "n3_iters": float(it) * n3}
 
     def cpu(seconds):
-        import ctypes as CC
         from oracle import oracle_py as O
         p = O.default_params(**params)
         t0 = time.perf_counter()
         done = 0
         for xf, sx in fix:
-            CC.CDLL(None).srand(2020)
+            C.CDLL(None).srand(2020)
             s = O.Solver(SPL, p)
             node = s.load_subproblem(xf, sx)
             s.evaluate(node, 1)
             done += 1
+            log(f"cpu oracle: node {done} bounded")
             if time.perf_counter() - t0 >= seconds:
                 break
         dt = time.perf_counter() - t0
-        return done / dt, f"{done} of the 64 nodes bounded by the oracle in {dt:.1f} s", None
+        return done / dt, f"the first {done} of the 64 nodes bounded by the oracle in {dt:.1f} s", None
 
-    cfg = {"workload": "config 3 batch: 15x15 torus, w +-1 seed 3, 64 nodes with 0..75 random "
-                       "fixings (n 150..225) in one batch, tri+pent cuts (TriIneq 1500, "
-                       "PentIneq 500), no hepta, no diff give-up",
+    cfg = {"workload": "64 nodes of the 15x15 torus (w +-1, seed 3) with 0..75 random fixings "
+                       "(n 150..225) in one batch, tri+pent cuts (TriIneq 1500, PentIneq 500), "
+                       "no hepta, no diff give-up, each node with its own incumbent",
            "n": "150..225", "nodes_per_step": 64}
     return cfg, step, cpu
 
@@ -150,23 +166,24 @@ def wl_c1(args):
     from madam_amd import instances as I
     SPL = I.objective_matrix(*I.read_rudy(os.path.join(ROOT, "tests", "golden", "rudy", "g05_60.0")))
 
-    def step(form):
-        B = G.Bounder(SPL, G.default_params(), projection=1 if form == "chosen" else 0, batch=16)
+    def step(form, sample=None):
+        kw = dict(CHOSEN, batch=16) if form == "chosen" else dict(PLAIN, batch=1)
+        B = G.Bounder(SPL, G.default_params(), **kw)
         opt = B.bab()
-        return {"bound": float(opt), "admm_iters": int(B.ctx.admm_iters), "nodes": int(B.ctx.nodes),
-                "n3_iters": float(B.ctx.admm_iters) * 60 ** 3, "root_bound": float(B.ctx.root_bound)}
+        return {"bound": float(opt), "bounds": np.array([opt]), "admm_iters": int(B.ctx.admm_iters),
+                "nodes": int(B.ctx.nodes), "n3_iters": float(B.ctx.admm_iters) * 60 ** 3,
+                "root_bound": float(B.ctx.root_bound)}
 
     def cpu(seconds):
-        import ctypes as CC
         from oracle import oracle_py as O
-        CC.CDLL(None).srand(2020)
+        C.CDLL(None).srand(2020)
         t0 = time.perf_counter()
         s = O.Solver(SPL, O.default_params())
         opt = s.bab()
         dt = time.perf_counter() - t0
         return s.s.nodes / dt, f"oracle B&B of g05_60.0: optimum {opt:.0f}, {s.s.nodes} nodes in {dt:.2f} s", opt
 
-    cfg = {"workload": "config 1: full B&B of g05_60.0 (G(60,0.5), w=1), params-file values",
+    cfg = {"workload": "full B&B of g05_60.0 (G(60,0.5), w=1), params-file values",
            "n": 60, "nodes_per_step": "B&B"}
     return cfg, step, cpu
 
@@ -182,7 +199,10 @@ def main():
     ap.add_argument("--workload", default="c2-root", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plain-nodes", type=int, default=4,
+                    help="c3-batch: nodes bounded one at a time by the plain form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the bounding path "
@@ -194,43 +214,57 @@ def main():
     lib = G.lib()
     cfg, step, cpu = WORKLOADS[args.workload](args)
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         step("chosen")
+        log(f"warmup step {w + 1} of {args.warmup} done")
     t0 = time.perf_counter()
     nodes = 0
-    for _ in range(args.steps):
+    for k in range(args.steps):
         r = step("chosen")
         nodes += r["nodes"]
+        log(f"timed step {k + 1} of {args.steps} done ({time.perf_counter() - t0:.2f} s)")
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
     value = nodes / dt
 
-    with Timer(lib) as tm:
-        rp = step("chosen")
-    eig = PHASES.index("eig")
-    eig_ms = float(tm.ms[eig])
-    flops = 2.0 * 42 * rp["n3_iters"]
-    achieved = flops / (eig_ms * 1e-3) / 1e12 if eig_ms > 0 else 0.0
-    roofline = {"bound": "mfma", "kernel": "k_ns_project (sign-iteration projection, FP32 MFMA "
-                "+ FP64 finish)", "achieved": achieved, "peak": PEAK["fp32-mfma"],
-                "unit": "TFLOP/s", "frac": achieved / PEAK["fp32-mfma"], "traffic": None,
-                "launches": int(tm.cnt[eig]), "avg_launch_ms": eig_ms / max(1, int(tm.cnt[eig])),
-                "flops_per_node_projection": "84 n^3 (39 FP32 + 3 FP64 products of order n)",
-                "projections": rp["admm_iters"],
-                "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
+    roofline = None
+    if not args.no_profile:
+        with Timer(lib) as tm:
+            rp = step("chosen")
+        log("profiled step done")
+        eig = PHASES.index("eig")
+        eig_ms = float(tm.ms[eig])
+        launches = int(tm.cnt[eig])
+        f32 = 2.0 * 39 * rp["n3_iters"]   # 39 FP32 products of order n per projection
+        f64 = 2.0 * 3 * rp["n3_iters"]    # 3 FP64 products
+        sec = eig_ms * 1e-3
+        achieved = (f32 + f64) / sec / 1e12 if sec > 0 else 0.0
+        t_peak = (f32 / PEAK_FP32_MFMA + f64 / PEAK_FP64_MFMA) / 1e12
+        roofline = {"bound": "mfma", "kernel": "k_ns_project (sign-iteration projection: 39 FP32-MFMA "
+                    "products + 3 FP64-MFMA products)", "achieved": achieved,
+                    "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+                    "frac": t_peak / sec if sec > 0 else 0.0, "traffic": None,
+                    "frac_note": "time at the FP32/FP64 MFMA peaks for these flops / measured time",
+                    "launches": launches, "avg_launch_ms": eig_ms / max(1, launches),
+                    "flops_per_projection": "84 n^3 (78 n^3 FP32 + 6 n^3 FP64)",
+                    "projections": rp["admm_iters"],
+                    "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
 
     plain = None
     if not args.no_plain:
+        sample = args.plain_nodes if args.workload == "c3-batch" else None
         t1 = time.perf_counter()
-        pr = step("plain")
+        pr = step("plain", sample)
         tp = time.perf_counter() - t1
-        plain = {"value": pr["nodes"] / tp, "unit": "node bounds/s", "s_per_step": tp,
-                 "bound": pr["bound"], "admm_iters": pr["admm_iters"],
-                 "chosen_over_plain": value / (pr["nodes"] / tp),
-                 "rel_bound_diff_chosen_vs_plain": abs(r["bound"] - pr["bound"]) / abs(pr["bound"])}
-        if args.workload == "c3-batch":
-            diff = np.abs(r["bounds"] - pr["bounds"]) / np.abs(pr["bounds"])
-            plain["rel_bound_diff_max"] = float(diff.max())
+        log("plain form done")
+        nb = len(pr["bounds"])
+        plain = {"value": pr["nodes"] / tp, "unit": "node bounds/s", "seconds": tp,
+                 "nodes": pr["nodes"], "bound": pr["bound"], "admm_iters": pr["admm_iters"],
+                 "chosen_over_plain": value / (pr["nodes"] / tp)}
+        diff = np.abs(r["bounds"][:nb] - pr["bounds"]) / np.abs(pr["bounds"])
+        plain["rel_bound_diff_max"] = float(diff.max())
+        if args.workload == "c1-bab":
+            plain["time_to_optimum_s"] = tp
 
     cpub = None
     if not args.no_cpu:
@@ -246,11 +280,11 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32-mfma+f64", "data": "synthetic", "config": cfg, "roofline": roofline,
             "plain_fp64_gpu": plain, "cpu_baseline": cpub,
-            "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"]}}
+            "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
     if args.workload == "c1-bab":
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]
-    print(json.dumps(line, default=float))
+    print(json.dumps(line, default=float), flush=True)
 
 
 if __name__ == "__main__":

@@ -217,6 +217,11 @@ typedef struct {
     double cg_rtol;                 /* PCG relative residual of the cut solve (<= 0: 1e-12) */
     long long cg_iters;             /* out: PCG iterations */
     long long ns_fallbacks;         /* out: sign-iteration projections redone by FP64 Jacobi */
+    int verbose;                    /* > 0: one stderr line per cutting-plane round / B&B batch */
+    int own_incumbent;              /* madam_sdpbound_batch: each node starts from LB and prunes
+                                     * against its own heuristic's best only, as separate
+                                     * SDPbound calls would; LB ends as the best of them.
+                                     * madam_bab ignores it (one shared incumbent). */
 } madam_bab_ctx;
 
 /* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n

@@ -360,7 +360,7 @@ class BabCtx(C.Structure):
         ("ipm_iters", C.c_longlong), ("open_nodes", C.c_longlong), ("root_bound", C.c_double),
         ("seconds_ipm", C.c_double), ("seconds_admm", C.c_double), ("seconds_sep", C.c_double),
         ("seconds_heur", C.c_double), ("cg_rtol", C.c_double), ("cg_iters", C.c_longlong),
-        ("ns_fallbacks", C.c_longlong),
+        ("ns_fallbacks", C.c_longlong), ("verbose", C.c_int), ("own_incumbent", C.c_int),
     ]
 
 
@@ -430,7 +430,8 @@ def gw_rounding(L, Z, num, seed=2020, stream=0):
 class Bounder:
     """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
 
-    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0, cg_rtol=0.0):
+    def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0, cg_rtol=0.0,
+                 verbose=0, own_incumbent=0):
         self.SPL = _f64(SPL)
         self.ctx = BabCtx()
         self.ctx.N = self.SPL.shape[0]
@@ -443,6 +444,8 @@ class Bounder:
         self.ctx.LB = LB
         self.ctx.projection = projection
         self.ctx.cg_rtol = cg_rtol
+        self.ctx.verbose = verbose
+        self.ctx.own_incumbent = own_incumbent
 
     def sdpbound(self, nodes, ranks=None):
         k = len(nodes)

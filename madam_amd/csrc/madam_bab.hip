@@ -10,6 +10,8 @@
 // host keeps what the reference keeps in process globals: the cut lists
 // (purged, deduplicated and appended here exactly as cutting_planes.c does),
 // the incumbent, diff and the parameters that grow on give-up.
+#include <stdio.h>
+
 #include <chrono>
 #include <unordered_set>
 
@@ -95,6 +97,7 @@ struct Run {
   int rank = 1, slot = 0, n = 0;
   double fixedvalue = 0, basic_bound = 0, bound = 0, oldf = 0, f = 0;
   double sigma = 1, tol = 0.05, viol3 = 0, viol5 = 0;
+  double lb = 0;  // this node's incumbent when ctx->own_incumbent is set
   int count = 0, nbit = 0, done = 0, giveup = 0, prune = 0;
   std::vector<double> PPL;
   std::vector<madam_tri> tri;
@@ -107,6 +110,12 @@ struct Ctx {
   madam_bab_ctx* c;
   Engine& E;
 };
+
+// The incumbent a node prunes against: the shared one (Bab_LBGet), or the
+// node's own with ctx->own_incumbent.
+inline double incumbent(const madam_bab_ctx* c, const Run* r) {
+  return c->own_incumbent ? r->lb : c->LB;
+}
 
 void store_fracsol(Run& r, int N, const std::vector<double>& lastcol) {
   int idx = 0;
@@ -175,9 +184,10 @@ int heuristic(madam_bab_ctx* c, std::vector<Run*>& runs, bool zero_planes) {
     int idx = 0;
     for (int i = 0; i < c->N - 1; ++i)
       sol[i] = r.node->xfixed[i] ? r.node->solX[i] : (z[idx++] + 1) / 2;
-    if (bv > c->LB) {
+    if (bv > incumbent(c, &r)) {
       // confirm on the original objective before taking it as the incumbent
       const double v = evaluate_solution(c->N, c->SPL, sol.data());
+      if (v > r.lb) r.lb = v;
       if (v > c->LB) {
         c->LB = v;
         for (int i = 0; i < c->N - 1; ++i) c->best_x[i] = sol[i];
@@ -495,6 +505,7 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
     r.fixedvalue = fixed_value(c->N, c->SPL, r.node->xfixed, r.node->solX);
     r.sigma = P.sigma0;
     r.tol = P.tol0;
+    r.lb = c->LB;
     all.push_back(&r);
   }
   double t0 = now_s();
@@ -508,11 +519,11 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
   std::vector<Run*> act;
   for (Run* r : all) {
     r->bound = r->f + r->fixedvalue;
-    if (r->bound < c->LB + 1.0) {
+    if (r->bound < incumbent(c, r) + 1.0) {
       r->done = 2;  // pruned before the loop: END without diff
       continue;
     }
-    if (P.use_diff && r->rank != 0 && r->bound > c->LB + c->diff + 1.0) {
+    if (P.use_diff && r->rank != 0 && r->bound > incumbent(c, r) + c->diff + 1.0) {
       r->done = 2;
       continue;
     }
@@ -523,11 +534,14 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
   if ((rc = separate_tri(c, act))) return rc;
   c->seconds_sep += now_s() - t0;
 
+  const double t_start = now_s();
+  int round = 0;
   while (!act.empty()) {
     for (Run* r : act) {
       ++r->count;
       r->oldf = r->f;
     }
+    ++round;
     t0 = now_s();
     if ((rc = admm_round(c, act))) return rc;
     c->seconds_admm += now_s() - t0;
@@ -536,7 +550,7 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
       r->tol *= P.scaleTol;
       if (r->tol < P.minTol) r->tol = P.minTol;
       r->bound = r->f + r->fixedvalue;
-      r->prune = r->bound < c->LB + 1.0;
+      r->prune = r->bound < incumbent(c, r) + 1.0;
       if (!r->prune) heur.push_back(r);
     }
     t0 = now_s();
@@ -544,8 +558,8 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
     c->seconds_heur += now_s() - t0;
     std::vector<Run*> sep;
     for (Run* r : act) {
-      if (!r->prune) r->prune = r->bound < c->LB + 1.0;
-      const double gap = r->bound - c->LB;
+      if (!r->prune) r->prune = r->bound < incumbent(c, r) + 1.0;
+      const double gap = r->bound - incumbent(c, r);
       if (r->count == P.triag_iter + P.pent_iter + P.hept_iter &&
           gap - 1.0 > (r->oldf - r->f) * (P.max_outer_iter - r->count))
         r->giveup = 1;
@@ -593,6 +607,17 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
       if (!r->done) next.push_back(r);
     }
     if ((rc = fetch_fracsol(act, c->N))) return rc;
+    if (c->verbose > 0) {
+      int nbmax = 0;
+      size_t mmax = 0;
+      for (Run* r : act) {
+        nbmax = std::max(nbmax, r->nbit);
+        mmax = std::max(mmax, r->tri.size() + r->pent.size() + r->hept.size());
+      }
+      fprintf(stderr, "sdpbound: round %d, %zu of %d nodes active, ADMM iterations <= %d, cuts <= %zu, %.2f s\n",
+              round, next.size(), k, nbmax, mmax, now_s() - t_start);
+      fflush(stderr);
+    }
     act.swap(next);
   }
   for (Run* r : all) {
@@ -710,6 +735,13 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
     return MADAM_E_ARG;
   }
   if (c->params.adjust_TriIneq) c->params.TriIneq = c->N * 10;
+  // one shared incumbent in the tree search, whatever own_incumbent says
+  struct Restore {
+    int& f;
+    int v;
+    ~Restore() { f = v; }
+  } restore{c->own_incumbent, c->own_incumbent};
+  c->own_incumbent = 0;
   const int batch = std::max(1, c->batch);
   madam_node* root = new madam_node();
   int rank0 = 0;
@@ -755,6 +787,11 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
         }
       }
       delete x;
+    }
+    if (c->verbose > 0) {
+      fprintf(stderr, "bab: %lld nodes bounded, %zu open, incumbent %.0f\n", c->nodes, h.d.size(),
+              c->LB);
+      fflush(stderr);
     }
   }
   c->open_nodes = (long long)h.d.size();
