@@ -1,6 +1,11 @@
 import os
 import sys
 
+# One BLAS thread for the oracle, as the reference runs (setupenv.sh:6-8):
+# its results then do not depend on the machine's core count.
+os.environ.setdefault("MKL_NUM_THREADS", "1")
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
