@@ -68,6 +68,8 @@ typedef struct {
     int check_every;       /* ADMM iterations between host checks of the stop flags, default 16 */
     int projection;        /* 0: FP64 eigen-decomposition (Jacobi); 1: mixed-precision sign
                               iteration (FP32 MFMA quintic Newton-Schulz + FP64 finish) */
+    int ns_layout;         /* sign iteration: 0 automatic, 1 one workgroup per node,
+                              2 tile-parallel (each product spread over many workgroups) */
 } madam_opts;
 
 /* One ADMM_solver call (admm.c:49): inputs are read, outputs written.
@@ -218,6 +220,7 @@ typedef struct {
     long long cg_iters;             /* out: PCG iterations */
     long long ns_fallbacks;         /* out: sign-iteration projections redone by FP64 Jacobi */
     int verbose;                    /* > 0: one stderr line per cutting-plane round / B&B batch */
+    int ns_layout;                  /* as madam_opts.ns_layout */
     int own_incumbent;              /* madam_sdpbound_batch: each node starts from LB and prunes
                                      * against its own heuristic's best only, as separate
                                      * SDPbound calls would; LB ends as the best of them.

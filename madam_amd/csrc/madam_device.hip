@@ -64,8 +64,9 @@ __device__ void spmv_bbt(const NodeView& nd, const double* __restrict__ p, doubl
   for (int c = threadIdx.x; c < nd.m; c += blockDim.x) {
     double acc = 0.0;
     for (int e = nd.cptr[c]; e < nd.cptr[c + 1]; ++e) {
-      const double v = w[nd.cact[e]];
-      acc = nd.cent[e] >= 0 ? acc + v : acc - v;
+      const int ca = nd.cact[e];
+      const double v = w[ent_idx(ca)];
+      acc = ca >= 0 ? acc + v : acc - v;
     }
     q[c] = p[c] + 0.5 * nd.cscale[c] * acc;
   }
@@ -129,6 +130,104 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int max
       const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
       p[c] = r[c] * dinv + beta * p[c];
     }
+    ++it;
+  }
+  if (threadIdx.x == 0) {
+    nd.cg_iters = it;
+    nd.cg_total += it;
+  }
+}
+
+// The same PCG with the cut-side vector ps = scale * v and the pair vector
+// w in LDS (when m + nact doubles fit): both passes of the product then
+// gather from LDS and stream only their index arrays (aent, cact) from HBM.
+// Same operations in the same order as k_cg, so the iterates are
+// bit-identical.
+__device__ inline void bbt_lds(const NodeView& nd, const double* __restrict__ v,
+                               double* __restrict__ out, double* ps, double* w) {
+  const int m = nd.m, na = nd.nact;
+  for (int c = threadIdx.x; c < m; c += WG) ps[c] = nd.cscale[c] * v[c];
+  __syncthreads();
+  for (int a = threadIdx.x; a < na; a += WG) {
+    double acc = 0.0;
+    const int e1 = nd.aptr[a + 1];
+    for (int e = nd.aptr[a]; e < e1; ++e) {
+      const int en = nd.aent[e];
+      const double x = ps[ent_idx(en)];
+      acc = en >= 0 ? acc + x : acc - x;
+    }
+    w[a] = acc;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < m; c += WG) {
+    double acc = 0.0;
+    const int e1 = nd.cptr[c + 1];
+    for (int e = nd.cptr[c]; e < e1; ++e) {
+      const int ca = nd.cact[e];
+      const double x = w[ent_idx(ca)];
+      acc = ca >= 0 ? acc + x : acc - x;
+    }
+    out[c] = v[c] + 0.5 * nd.cscale[c] * acc;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int maxit) {
+  extern __shared__ double smem[];
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done) return;
+  const int m = nd.m;
+  if (m == 0) return;
+  double* ps = smem;
+  double* w = smem + m;
+  double* red = w + nd.nact;
+  double* __restrict__ x = nd.t;
+  double* __restrict__ r = nd.cg_r;
+  double* __restrict__ p = nd.cg_p;
+  double* __restrict__ q = nd.cg_q;
+  const double* __restrict__ b = nd.rhs;
+  bbt_lds(nd, x, q, ps, w);
+  double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
+  for (int c = threadIdx.x; c < m; c += WG) {
+    const int k = nd.cptr[c + 1] - nd.cptr[c];
+    const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
+    const double rc = b[c] - q[c];
+    r[c] = rc;
+    p[c] = rc * dinv;
+    v3[0] += rc * rc * dinv;
+    v3[1] += rc * rc;
+    v3[2] += b[c] * b[c];
+  }
+  block_sums<3>(v3, red);
+  double rz = v3[0], rr = v3[1];
+  const double bb = v3[2];
+  const double thr = rtol * rtol * bb;
+  int it = 0;
+  while (it < maxit && rr > thr && bb > 0.0) {
+    bbt_lds(nd, p, q, ps, w);
+    double v1[1] = {0.0};
+    for (int c = threadIdx.x; c < m; c += WG) v1[0] += p[c] * q[c];
+    block_sums<1>(v1, red);
+    const double alpha = rz / v1[0];
+    double v2[2] = {0, 0};
+    for (int c = threadIdx.x; c < m; c += WG) {
+      const int k = nd.cptr[c + 1] - nd.cptr[c];
+      const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
+      x[c] += alpha * p[c];
+      const double rc = r[c] - alpha * q[c];
+      r[c] = rc;
+      v2[0] += rc * rc * dinv;
+      v2[1] += rc * rc;
+    }
+    block_sums<2>(v2, red);
+    const double beta = v2[0] / rz;
+    rz = v2[0];
+    rr = v2[1];
+    for (int c = threadIdx.x; c < m; c += WG) {
+      const int k = nd.cptr[c + 1] - nd.cptr[c];
+      const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
+      p[c] = r[c] * dinv + beta * p[c];
+    }
+    __syncthreads();
     ++it;
   }
   if (threadIdx.x == 0) {
@@ -602,8 +701,13 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   k_admm_prep<<<grid, WGS, 0, st>>>(nodes);
 }
 
-void launch_cg(NodeView* nodes, int nb, double rtol, int maxit, hipStream_t st) {
-  k_cg<<<nb, WG, 0, st>>>(nodes, rtol, maxit);
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int maxit,
+               size_t lds_limit, hipStream_t st) {
+  const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
+  if (lds <= lds_limit)
+    k_cg_lds<<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+  else
+    k_cg<<<nb, WG, 0, st>>>(nodes, rtol, maxit);
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

@@ -143,6 +143,10 @@ inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const 
       const int a = cs.cact[e];
       cs.aent[fill[a]++] = cs.cent[e] >= 0 ? c : ~c;
     }
+  // the entry's sign travels with its active-pair id (a or ~a), so that the
+  // cut-side pass of the BB^T product reads one index stream
+  for (size_t e = 0; e < cs.cent.size(); ++e)
+    if (cs.cent[e] < 0) cs.cact[e] = ~cs.cact[e];
   return MADAM_OK;
 }
 
@@ -296,6 +300,7 @@ struct Engine {
   int device = -1;
   hipStream_t stream = nullptr;
   size_t lds_limit = 0;
+  int cu_count = 256;
   std::vector<Slot> slots;
   DBuf<NodeView> views;
   DBuf<double> tmpd;
@@ -330,6 +335,7 @@ inline madam_opts default_opts() {
   o.jacobi_sweeps = 40;
   o.check_every = 16;
   o.projection = 0;
+  o.ns_layout = 0;
   return o;
 }
 

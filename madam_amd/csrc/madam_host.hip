@@ -40,22 +40,33 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
   if ((rc = E.views.ensure(k))) return rc;
   if ((rc = up(E.views.p, hv.data(), k, st))) return rc;
   const int chunk = std::max(1, o.check_every);
-  int itmax_all = 0;
-  for (int b = 0; b < k; ++b) itmax_all = std::max(itmax_all, hv[b].itermax);
+  int itmax_all = 0, namax = 1;
+  for (int b = 0; b < k; ++b) {
+    itmax_all = std::max(itmax_all, hv[b].itermax);
+    namax = std::max(namax, hv[b].nact);
+  }
   long launched = 0;
+  int active = k;
   for (;;) {
     if (launched > (long)itmax_all + 2 * chunk + 8) {
       set_err("ADMM stop flags never set after %ld iterations", launched);
       return MADAM_E_HIP;
     }
     launched += chunk;
+    // tile-parallel sign iteration while the active nodes cannot fill the
+    // CUs with one workgroup each
+    const bool tiles = o.ns_layout == 2 || (o.ns_layout == 0 && active < E.cu_count / 2);
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
-      TIMED(PH_CG, st, madam::launch_cg(E.views.p, k, o.cg_rtol, o.cg_maxit, st));
+      TIMED(PH_CG, st,
+            madam::launch_cg(E.views.p, k, mmax, namax, o.cg_rtol, o.cg_maxit, E.lds_limit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
         TIMED(PH_EIG, st, {
-          madam::launch_ns_project(E.views.p, k, 0, st);
+          if (tiles)
+            madam::launch_ns_tiles(E.views.p, k, nmax, st);
+          else
+            madam::launch_ns_project(E.views.p, k, 0, st);
           madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 3);
         });
       } else {
@@ -68,9 +79,9 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     E.timer.flush();
-    bool all = true;
-    for (int b = 0; b < k; ++b) all = all && hv[b].done;
-    if (all) break;
+    active = 0;
+    for (int b = 0; b < k; ++b) active += hv[b].done ? 0 : 1;
+    if (active == 0) break;
   }
   if (o.projection == 1) {
     // eigen-pairs of the last W for the rounding heuristic's factor Z
@@ -132,6 +143,7 @@ int madam_open(int device) {
   Engine* e = new Engine();
   e->device = device;
   e->lds_limit = prop.sharedMemPerBlock;
+  e->cu_count = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     delete e;
     set_err("hipStreamCreate failed");

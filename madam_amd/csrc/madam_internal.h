@@ -37,7 +37,7 @@ struct NodeView {
 
   const int *cptr;      // m+1
   const int *cent;      // cptr[m]: lo or ~lo
-  const int *cact;      // cptr[m]: active-pair id of the entry
+  const int *cact;      // cptr[m]: active-pair id a of the entry, as a or ~a like cent
   const double *cscale; // m
   const int *aptr;      // nact+1
   const int *aent;      // aptr[nact]: c or ~c
@@ -66,7 +66,8 @@ struct NodeView {
 // Launch helpers implemented in madam_device.hip.  All take the device array
 // of nodes and the batch size, and run on `stream`.
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
-void launch_cg(NodeView* nodes, int nb, double rtol, int maxit, hipStream_t stream);
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int maxit,
+               size_t lds_limit, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
                    int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
@@ -80,6 +81,7 @@ void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t stream);
 void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t stream);
+void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, const double* A64,
                       const double* B64, double* C64, int n, double alpha, double beta,
                       hipStream_t stream);

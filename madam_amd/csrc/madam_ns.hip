@@ -27,6 +27,8 @@
 #include "madam_internal.h"
 #include "madam_ns_coef.h"
 
+#include <algorithm>
+
 namespace madam {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -40,75 +42,92 @@ constexpr int NS_STEPS = MADAM_NS_STEPS;
 // Lane layout of v_mfma_f32_16x16x4f32: A operand lane l -> (row l%16,
 // k l/16); B operand lane l -> (k l/16, col l%16); result lane l -> rows
 // 4(l/16)+v, col l%16.
-__device__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
-                             float* __restrict__ Cm, int np, float alpha, float beta) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int nt = np >> 4;
-  const int ntile = nt * (nt + 1) / 2;
+__device__ inline void tile_decode(int t, int& i0, int& j0) {
+  int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  if (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  i0 = ti << 4;
+  j0 = tj << 4;
+}
+
+// One lower 16x16 tile t of C = A (alpha I + beta B), computed by the
+// calling wave.
+__device__ inline void gemm_tile_f32(const float* __restrict__ A, const float* __restrict__ B,
+                                     float* __restrict__ Cm, int np, float alpha, float beta,
+                                     int t) {
+  const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  for (int t = wave; t < ntile; t += nw) {
-    int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    if (ti * (ti + 1) / 2 > t) --ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int i0 = ti << 4, j0 = tj << 4;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < np; k0 += 4) {
-      const int k = k0 + q;
-      const float a = A[(i0 + r) + (size_t)k * np];
-      // B symmetric: B[k][j] = B[j + k np]
-      float b = beta * B[(j0 + r) + (size_t)k * np];
-      if (j0 + r == k) b += alpha;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-    }
-    // only the lower element of each (i,j)/(j,i) pair is stored, to both
-    // places: the iterates stay exactly symmetric, which the transposed
-    // operand reads rely on (an asymmetric part would grow step by step)
+  int i0, j0;
+  tile_decode(t, i0, j0);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k0 = 0; k0 < np; k0 += 4) {
+    const int k = k0 + q;
+    const float a = A[(i0 + r) + (size_t)k * np];
+    // B symmetric: B[k][j] = B[j + k np]
+    float b = beta * B[(j0 + r) + (size_t)k * np];
+    if (j0 + r == k) b += alpha;
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  // only the lower element of each (i,j)/(j,i) pair is stored, to both
+  // places: the iterates stay exactly symmetric, which the transposed
+  // operand reads rely on (an asymmetric part would grow step by step)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + 4 * q + v, j = j0 + r;
-      if (i >= j) {
-        Cm[i + (size_t)j * np] = acc[v];
-        Cm[j + (size_t)i * np] = acc[v];
-      }
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + 4 * q + v, j = j0 + r;
+    if (i >= j) {
+      Cm[i + (size_t)j * np] = acc[v];
+      Cm[j + (size_t)i * np] = acc[v];
     }
   }
+}
+
+__device__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
+                             float* __restrict__ Cm, int np, float alpha, float beta) {
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nt = np >> 4;
+  const int ntile = nt * (nt + 1) / 2;
+  for (int t = wave; t < ntile; t += nw) gemm_tile_f32(A, B, Cm, np, alpha, beta, t);
 }
 
 // Same in FP64 with v_mfma_f64_16x16x4f64, on n x n matrices of any n
 // (tiles past n are masked).  op: C = A (alpha I + beta B).  Operand
 // layouts as in the FP32 case; the result register v of lane l holds row
 // (l/16) + 4v, column l%16 (measured on gfx950).
-__device__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
-                             double* __restrict__ Cm, int n, double alpha, double beta) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int nt = (n + 15) >> 4;
-  const int ntile = nt * (nt + 1) / 2;
+__device__ inline void gemm_tile_f64(const double* __restrict__ A, const double* __restrict__ B,
+                                     double* __restrict__ Cm, int n, double alpha, double beta,
+                                     int t) {
+  const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  for (int t = wave; t < ntile; t += nw) {
-    int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    if (ti * (ti + 1) / 2 > t) --ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int i0 = ti << 4, j0 = tj << 4;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += 4) {
-      const int k = k0 + q;
-      const int ia = i0 + r, jb = j0 + r;
-      const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
-      double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
-      if (jb == k && k < n) b += alpha;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
+  int i0, j0;
+  tile_decode(t, i0, j0);
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int ia = i0 + r, jb = j0 + r;
+#pragma unroll 4
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int k = k0 + q;
+    const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
+    double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
+    if (jb == k && k < n) b += alpha;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + q + 4 * v, j = j0 + r;
-      if (i < n && j < n && i >= j) {
-        Cm[i + (size_t)j * n] = acc[v];
-        Cm[j + (size_t)i * n] = acc[v];
-      }
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + q + 4 * v, j = j0 + r;
+    if (i < n && j < n && i >= j) {
+      Cm[i + (size_t)j * n] = acc[v];
+      Cm[j + (size_t)i * n] = acc[v];
     }
   }
+}
+
+__device__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+                             double* __restrict__ Cm, int n, double alpha, double beta) {
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nt = (n + 15) >> 4;
+  const int ntile = nt * (nt + 1) / 2;
+  for (int t = wave; t < ntile; t += nw) gemm_tile_f64(A, B, Cm, n, alpha, beta, t);
 }
 
 // Projection of nd.W (FP64) -> X_+ in nd.S1.  ns32: 3 n_p^2 floats.
@@ -182,6 +201,102 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
 
 void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t st) {
   k_ns_project<<<nb, 1024, 0, st>>>(nodes, ignore_done);
+}
+
+// ---------------------------------------------------------------------------
+// Tile-parallel form of the same projection: one launch per product, one
+// wave per lower 16x16 tile, grid (tile groups, nodes), so that a node's
+// product is spread over many CUs.  Used when the batch has fewer nodes
+// than the chip has CUs (a single node, or the stragglers of a batch); the
+// arithmetic is the one of k_ns_project, tile for tile.
+
+constexpr int NS_TW = 4;  // waves (= tiles) per workgroup
+
+// S_0 = W / |W|_F into buffer 0 of ns32 (zero padded), clear ns_fail.
+__global__ __launch_bounds__(1024) void k_ns_init(NodeView* nodes) {
+  __shared__ double red[64];
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done) return;
+  const int n = nd.n, np = (n + 15) & ~15, tid = threadIdx.x;
+  const size_t npp = (size_t)np * np;
+  const double* __restrict__ Wm = nd.W;
+  double fv[1] = {0.0};
+  for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
+  block_sums<1>(fv, red);
+  const double fro = sqrt(fv[0]);
+  const float inv = fro > 0.0 ? (float)(1.0 / fro) : 0.f;
+  float* Sb = nd.ns32;
+  for (size_t f = tid; f < npp; f += WG) {
+    const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
+    Sb[f] = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
+  }
+  if (tid == 0) nd.ns_fail = 0;
+}
+
+// C = A (alpha I + beta B) on buffers ia, ib, ic of ns32.
+__global__ __launch_bounds__(64 * NS_TW) void k_ns_gemm32(NodeView* nodes, int ia, int ib, int ic,
+                                                          float alpha, float beta) {
+  NodeView& nd = nodes[blockIdx.y];
+  if (nd.done) return;
+  const int np = (nd.n + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
+  const int t = blockIdx.x * NS_TW + (threadIdx.x >> 6);
+  if (t >= ntile) return;
+  const size_t npp = (size_t)np * np;
+  float* base = nd.ns32;
+  gemm_tile_f32(base + ia * npp, base + ib * npp, base + ic * npp, np, alpha, beta, t);
+}
+
+// FP32 iterate (buffer is) -> FP64 n x n in S2, with the guard.
+__global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int is) {
+  NodeView& nd = nodes[blockIdx.y];
+  if (nd.done) return;
+  const int n = nd.n, np = (n + 15) & ~15;
+  const float* S = nd.ns32 + (size_t)is * np * np;
+  int bad = 0;
+  for (int f = blockIdx.x * 256 + threadIdx.x; f < n * n; f += gridDim.x * 256) {
+    const int j = f / n, i = f - j * n;
+    const float v = S[i + (size_t)j * np];
+    if (!(fabsf(v) <= 1.5f)) bad = 1;
+    nd.S2[f] = (double)v;
+  }
+  if (bad) nd.ns_fail = 1;  // every writer stores the same value
+}
+
+// The FP64 finish, one product per launch: 0: S1 = S2^2;
+// 1: ip1 = S2 (1.5 I - 0.5 S1); 2: S1 = W (0.5 I + 0.5 ip1) = X_+.
+__global__ __launch_bounds__(64 * NS_TW) void k_ns_gemm64(NodeView* nodes, int which) {
+  NodeView& nd = nodes[blockIdx.y];
+  if (nd.done || nd.ns_fail) return;
+  const int n = nd.n, nt = (n + 15) >> 4, ntile = nt * (nt + 1) / 2;
+  const int t = blockIdx.x * NS_TW + (threadIdx.x >> 6);
+  if (t >= ntile) return;
+  if (which == 0)
+    gemm_tile_f64(nd.S2, nd.S2, nd.S1, n, 0.0, 1.0, t);
+  else if (which == 1)
+    gemm_tile_f64(nd.S2, nd.S1, nd.ip1, n, 1.5, -0.5, t);
+  else
+    gemm_tile_f64(nd.W, nd.ip1, nd.S1, n, 0.5, 0.5, t);
+}
+
+void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
+  static const float coef[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
+  const int np = (nmax + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
+  const dim3 g32((ntile + NS_TW - 1) / NS_TW, nb);
+  k_ns_init<<<nb, 1024, 0, st>>>(nodes);
+  int sb = 0, tb = 1, ub = 2;
+  for (int k = 0; k < MADAM_NS_STEPS; ++k) {
+    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, sb, sb, tb, 0.f, 1.f);               // T = S^2
+    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, tb, tb, ub, coef[k][1], coef[k][2]);  // U
+    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, sb, ub, tb, coef[k][0], 1.f);         // S'
+    const int x = sb;
+    sb = tb;
+    tb = x;
+  }
+  const int nconv = std::min(64, (nmax * nmax + 255) / 256);
+  k_ns_conv<<<dim3(nconv, nb), 256, 0, st>>>(nodes, sb);
+  const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
+  const dim3 g64((ntile64 + NS_TW - 1) / NS_TW, nb);
+  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 64 * NS_TW, 0, st>>>(nodes, w);
 }
 
 // Self-test of the MFMA operand/result layouts: C32 = A (alpha I + beta B)

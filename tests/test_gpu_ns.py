@@ -103,3 +103,18 @@ def test_sdpbound_root_ns_stable_across_cut_solves(cg_rtol):
     assert abs(bg - bp) <= 2e-3 * abs(bp)
     assert B.ctx.ns_fallbacks == 0
     assert B.ctx.admm_iters < 2000
+
+
+@pytest.mark.parametrize("graph,kgon", [("g05_60.0", False), ("w01_100.3", True)])
+def test_tile_parallel_layout_is_bit_identical(graph, kgon):
+    """The tile-parallel launch sequence computes the same tiles with the
+    same instruction order as the one-workgroup-per-node kernel: the ADMM
+    results must agree bit for bit."""
+    sol, case = admm_case(graph_L(graph), kgon=kgon)
+    case["tol"] = 1e-3
+    a = G.ADMM_solver(**case, projection=1, ns_layout=1)
+    b = G.ADMM_solver(**case, projection=1, ns_layout=2)
+    print(graph, "layout1", a.f, a.nbit, "layout2", b.f, b.nbit)
+    assert a.nbit == b.nbit
+    assert a.f == b.f
+    assert np.array_equal(a.X, b.X)
