@@ -324,6 +324,7 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
   if (form_xplus == 3) {
     // fallback behind the sign-iteration projection: only its failed nodes
     if (!nd.ns_fail) return;
+    if (threadIdx.x == 0) nd.ns_fallbacks += 1;
     form_xplus = 1;
   }
   const int n = nd.n;

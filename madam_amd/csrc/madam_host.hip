@@ -55,7 +55,10 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     launched += chunk;
     // tile-parallel sign iteration while the active nodes cannot fill the
     // CUs with one workgroup each
-    const bool tiles = o.ns_layout == 2 || (o.ns_layout == 0 && active < E.cu_count / 2);
+    // (nodes of order <= 112 keep their iterates in LDS: one workgroup each)
+    const bool small = madam::ns_lds_bytes(nmax) + 1024 <= E.lds_limit;
+    const bool tiles =
+        o.ns_layout == 2 || (o.ns_layout == 0 && !small && active < E.cu_count / 2);
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
@@ -66,7 +69,7 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
           if (tiles)
             madam::launch_ns_tiles(E.views.p, k, nmax, st);
           else
-            madam::launch_ns_project(E.views.p, k, 0, st);
+            madam::launch_ns_project(E.views.p, k, nmax, 0, E.lds_limit, st);
           madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 3);
         });
       } else {
@@ -567,7 +570,7 @@ int madam_project_sdp_ns(int n, const double* W, double* X, double* Q, int* M) {
   const size_t nn = (size_t)n * n;
   if ((rc = up(g_eng->views.p, g_eng->hviews.data(), 1, st)) || (rc = up(sl.W.p, W, nn, st))) return rc;
   TIMED(PH_EIG, st, {
-    madam::launch_ns_project(g_eng->views.p, 1, 1, st);
+    madam::launch_ns_project(g_eng->views.p, 1, n, 1, g_eng->lds_limit, st);
     madam::launch_jacobi(g_eng->views.p, 1, n, 0, 40, 1, g_eng->lds_limit, st, 3);
   });
   madam::launch_project(g_eng->views.p, 1, n, st);

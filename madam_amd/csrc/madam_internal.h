@@ -80,7 +80,9 @@ void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout
                   hipStream_t stream);
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t stream);
-void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t stream);
+void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_t lds_limit,
+                       hipStream_t stream);
+size_t ns_lds_bytes(int nmax);
 void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, const double* A64,
                       const double* B64, double* C64, int n, double alpha, double beta,

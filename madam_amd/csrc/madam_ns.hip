@@ -16,9 +16,12 @@
 // (v_mfma_f64_16x16x4f64) restores S^2 = I to FP64 precision, and the
 // projection itself, W_+ = (W + W S) / 2, is an FP64 product.
 //
-// One workgroup per node; the FP32 iterates live in a per-node scratch
-// (3 n_p^2 floats, n_p = n rounded up to 16, zero padded), FP64 matrices in
-// the node's S1/S2.  Output: X_+ in S1 (full), M = round(trace((I+S)/2)).
+// Two launch layouts with identical arithmetic: one workgroup per node
+// (k_ns_project; the FP32 iterates, 3 n_p^2 floats with n_p = n rounded up
+// to 16 and zero padded, live in LDS when n_p <= 112, else in the node's
+// ns32 scratch), or one launch per product spread over many workgroups
+// (k_ns_gemm32 and friends, below) for batches that cannot fill the CUs.
+// FP64 matrices live in the node's S1/S2.  Output: X_+ in S1 (full), M = round(trace((I+S)/2)).
 // Guard: an FP32 iterate with an entry above 1.5 in magnitude or not finite
 // (a sign matrix has all entries in [-1, 1]) sets nd.ns_fail and leaves S1
 // for the FP64 Jacobi projection that the host launches behind this kernel
@@ -53,7 +56,7 @@ __device__ inline void tile_decode(int t, int& i0, int& j0) {
 
 // One lower 16x16 tile t of C = A (alpha I + beta B), computed by the
 // calling wave.
-__device__ inline void gemm_tile_f32(const float* __restrict__ A, const float* __restrict__ B,
+__device__ __forceinline__ void gemm_tile_f32(const float* __restrict__ A, const float* __restrict__ B,
                                      float* __restrict__ Cm, int np, float alpha, float beta,
                                      int t) {
   const int lane = threadIdx.x & 63;
@@ -83,7 +86,7 @@ __device__ inline void gemm_tile_f32(const float* __restrict__ A, const float* _
   }
 }
 
-__device__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
+__device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
                              float* __restrict__ Cm, int np, float alpha, float beta) {
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nt = np >> 4;
@@ -95,7 +98,7 @@ __device__ void sym_gemm_f32(const float* __restrict__ A, const float* __restric
 // (tiles past n are masked).  op: C = A (alpha I + beta B).  Operand
 // layouts as in the FP32 case; the result register v of lane l holds row
 // (l/16) + 4v, column l%16 (measured on gfx950).
-__device__ inline void gemm_tile_f64(const double* __restrict__ A, const double* __restrict__ B,
+__device__ __forceinline__ void gemm_tile_f64(const double* __restrict__ A, const double* __restrict__ B,
                                      double* __restrict__ Cm, int n, double alpha, double beta,
                                      int t) {
   const int lane = threadIdx.x & 63;
@@ -122,7 +125,7 @@ __device__ inline void gemm_tile_f64(const double* __restrict__ A, const double*
   }
 }
 
-__device__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+__device__ __forceinline__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
                              double* __restrict__ Cm, int n, double alpha, double beta) {
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nt = (n + 15) >> 4;
@@ -130,14 +133,18 @@ __device__ void sym_gemm_f64(const double* __restrict__ A, const double* __restr
   for (int t = wave; t < ntile; t += nw) gemm_tile_f64(A, B, Cm, n, alpha, beta, t);
 }
 
-// Projection of nd.W (FP64) -> X_+ in nd.S1.  ns32: 3 n_p^2 floats.
+// Projection of nd.W (FP64) -> X_+ in nd.S1, one workgroup per node.  The
+// three FP32 iterates live in LDS when they fit (3 n_p^2 floats, n_p <= 112:
+// operand reads at LDS latency), else in the node's ns32 scratch.
+template <bool LDS>
 __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore_done) {
+  extern __shared__ float sm32[];
   __shared__ double red[64];
   NodeView& nd = nodes[blockIdx.x];
   if (!ignore_done && nd.done) return;
   const int n = nd.n, np = (n + 15) & ~15, tid = threadIdx.x;
   const size_t npp = (size_t)np * np;
-  float* Sb = nd.ns32;
+  float* Sb = LDS ? sm32 : nd.ns32;
   float* Tb = Sb + npp;
   float* Ub = Tb + npp;
   const double* __restrict__ Wm = nd.W;
@@ -180,10 +187,7 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   }
   __syncthreads();
   if (bad) {
-    if (tid == 0) {
-      nd.ns_fail = 1;
-      nd.ns_fallbacks += 1;
-    }
+    if (tid == 0) nd.ns_fail = 1;
     return;
   }
   if (tid == 0) nd.ns_fail = 0;
@@ -199,8 +203,18 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   if (tid == 0) nd.M = (int)lrint(0.5 * (n + tr[0]));
 }
 
-void launch_ns_project(NodeView* nodes, int nb, int ignore_done, hipStream_t st) {
-  k_ns_project<<<nb, 1024, 0, st>>>(nodes, ignore_done);
+size_t ns_lds_bytes(int nmax) {
+  const size_t np = (size_t)((nmax + 15) & ~15);
+  return 3 * np * np * sizeof(float);
+}
+
+void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_t lds_limit,
+                       hipStream_t st) {
+  const size_t lds = ns_lds_bytes(nmax);
+  if (lds + 1024 <= lds_limit)
+    k_ns_project<true><<<nb, 1024, lds, st>>>(nodes, ignore_done);
+  else
+    k_ns_project<false><<<nb, 1024, 0, st>>>(nodes, ignore_done);
 }
 
 // ---------------------------------------------------------------------------
