@@ -188,7 +188,47 @@ def wl_c1(args):
     return cfg, step, cpu
 
 
-WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1}
+def wl_c4(args):
+    """Learned warm start: 2000 training graphs G(100, 0.5), w in {-1,0,1}
+    (targets from the GPU FP64 ADMM, untimed), then the step = warm-started
+    ADMM (basic SDP, tol 1e-5) on 200 fresh graphs, batched; the plain form
+    is the cold start; both report ADMM iterations."""
+    import madam_amd as G
+    from madam_amd import instances as I
+    from madam_amd import learned as LW
+    t0 = time.perf_counter()
+    Ltr = [I.objective_matrix(*I.config4_graph(10000 + s)) for s in range(args.c4_train)]
+    Lev = [I.objective_matrix(*I.config4_graph(20000 + s)) for s in range(200)]
+    Xs = [r.X for r in LW.solve_basic(Ltr, tol=1e-6)]
+    log(f"c4: targets for {len(Ltr)} graphs in {time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    model, losses = LW.LearnedWarmStart.train(Ltr, Xs, epochs=2, batch=4096, lr=2e-3, seed=0)
+    log(f"c4: {len(losses)} training steps on the GPU in {time.perf_counter() - t1:.1f} s, "
+        f"loss {losses[:20].mean():.4f} -> {losses[-20:].mean():.4f}")
+    info = {"train_graphs": len(Ltr), "train_steps": int(len(losses)),
+            "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-20:].mean())}
+
+    def step(form, sample=None):
+        if form == "chosen":
+            starts = [model.start(L) for L in Lev]
+            res = LW.solve_basic(Lev, tol=1e-5, starts=starts)
+        else:
+            res = LW.solve_basic(Lev, tol=1e-5)
+        bs = np.array([r.f for r in res])
+        it = sum(r.nbit for r in res)
+        return {"bound": float(bs.mean()), "bounds": bs, "admm_iters": it, "nodes": len(Lev),
+                "n3_iters": float(it) * 100 ** 3}
+
+    def cpu(seconds):
+        return None, "not timed for this workload", None
+
+    cfg = {"workload": "learned warm start: 2000 training graphs G(100,0.5), w in {-1,0,1}; "
+                       "basic-SDP ADMM (tol 1e-5) on 200 fresh graphs, warm vs cold start",
+           "n": 100, "nodes_per_step": 200, **info}
+    return cfg, step, cpu
+
+
+WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4}
 
 
 def main():
@@ -201,6 +241,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--c4-train", type=int, default=2000, help="c4-learned: training graphs")
     ap.add_argument("--plain-nodes", type=int, default=4,
                     help="c3-batch: nodes bounded one at a time by the plain form")
     args = ap.parse_args()
@@ -267,7 +308,7 @@ def main():
             plain["time_to_optimum_s"] = tp
 
     cpub = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload != "c4-learned":
         cval, sample, cbound = cpu(args.cpu_seconds)
         cpub = {"value": cval, "unit": "node bounds/s", "cores": 1, "kind": "port",
                 "sample": sample + " (1 thread, MKL_NUM_THREADS=1)"}
@@ -284,6 +325,9 @@ def main():
     if args.workload == "c1-bab":
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]
+    if args.workload == "c4-learned" and plain is not None:
+        line["admm_iterations"] = {"warm": r["admm_iters"], "cold": plain["admm_iters"],
+                                   "saved_frac": 1.0 - r["admm_iters"] / plain["admm_iters"]}
     print(json.dumps(line, default=float), flush=True)
 
 

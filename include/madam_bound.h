@@ -244,6 +244,40 @@ int madam_sdpbound_batch(madam_bab_ctx *ctx, int k, madam_node *const *nodes, co
  * *optimum = ctx->LB at the end (the proven optimum when open_nodes = 0). */
 int madam_bab(madam_bab_ctx *ctx, long long max_nodes, double *optimum);
 
+
+/* ---- Learned warm start (no reference counterpart: the reference starts
+ * each node's ADMM from its interior-point solution).  A small MLP maps the
+ * features of each vertex pair (i, j) of a node's cost matrix to a
+ * prediction of the converged ADMM primal X*_ij; the prediction, made PSD
+ * with unit diagonal, and the dual y_i = (L X)_ii with Q = (Diag(y) - L)_+
+ * that complementarity gives, warm-start ADMM_solver.  Cold-start ADMM
+ * stays the fallback: the safe bound does not depend on the start. */
+#define MADAM_NFEAT 10
+typedef struct {
+    int nin;        /* features per pair (MADAM_NFEAT) */
+    int nh;         /* hidden width, must be 64 */
+    float *params;  /* FP32 master parameters, madam_mlp_nparams(nin) floats:
+                       W1[nin][64], b1[64], W2[64][64], b2[64], W3[64], b3 */
+} madam_mlp;
+int madam_mlp_nparams(int nin);
+/* Pair features of L (n x n): n(n-1)/2 rows of MADAM_NFEAT floats, pairs
+ * (0,1), (0,2), ..., (1,2), ...; standardised by mu/sd when both given. */
+int madam_pair_features(int n, const double *L, const float *mu, const float *sd, float *feat);
+/* out[s] = network(feat[s]) for B samples (BF16 weights, FP32 sums). */
+int madam_mlp_forward(const madam_mlp *net, int B, const float *feat, float *out);
+/* Gradient of mean((out - target)^2 / 2) over B samples, and that loss. */
+int madam_mlp_grad(const madam_mlp *net, int B, const float *feat, const float *target,
+                   float *grad, float *loss);
+/* Adam training on the GPU over N samples: epochs x (N / batch) steps of
+ * shuffled minibatches (order from seed); net->params updated in place;
+ * losses (may be null) receives each step's minibatch loss. */
+int madam_mlp_train(madam_mlp *net, long long N, const float *feat, const float *target,
+                    int epochs, int batch, float lr, unsigned long long seed, float *losses);
+/* Warm start for ADMM_solver from predicted pair values pred (n(n-1)/2):
+ * X = the PSD part of the prediction, rescaled to unit diagonal; y, Q as
+ * above.  X, Q n x n, y n. */
+int madam_warm_start(int n, const double *L, const float *pred, double *X, double *y, double *Q);
+
 #ifdef __cplusplus
 }
 #endif

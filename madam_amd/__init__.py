@@ -492,3 +492,100 @@ def mfma_selftest(A32, B32, A64, B64, alpha, beta):
                                      A64.ctypes.data_as(_PD), B64.ctypes.data_as(_PD),
                                      C64.ctypes.data_as(_PD), C.c_double(alpha), C.c_double(beta)))
     return C32, C64
+
+
+# ------------------------------------------------------------------ learned warm start
+
+NFEAT = 10
+
+
+class Mlp(C.Structure):
+    _fields_ = [("nin", C.c_int), ("nh", C.c_int), ("params", C.POINTER(C.c_float))]
+
+
+_PF = C.POINTER(C.c_float)
+
+
+def mlp_nparams(nin=NFEAT):
+    return lib().madam_mlp_nparams(C.c_int(nin))
+
+
+def mlp_init(nin=NFEAT, seed=0):
+    """FP32 parameters: Glorot-uniform weights, zero biases."""
+    rng = np.random.default_rng(seed)
+    H = 64
+    parts = [rng.uniform(-1, 1, nin * H) * np.sqrt(6.0 / (nin + H)), np.zeros(H),
+             rng.uniform(-1, 1, H * H) * np.sqrt(6.0 / (2 * H)), np.zeros(H),
+             rng.uniform(-1, 1, H) * np.sqrt(6.0 / (H + 1)), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == mlp_nparams(nin)
+    return p
+
+
+def _net(params, nin):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    net = Mlp()
+    net.nin, net.nh, net.params = nin, 64, params.ctypes.data_as(_PF)
+    return net, params
+
+
+def mlp_forward(params, feat):
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    net, keep = _net(params, feat.shape[1])
+    out = np.zeros(feat.shape[0], np.float32)
+    _check(lib().madam_mlp_forward(C.byref(net), C.c_int(feat.shape[0]), feat.ctypes.data_as(_PF),
+                                   out.ctypes.data_as(_PF)))
+    return out
+
+
+def mlp_grad(params, feat, target):
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    net, keep = _net(params, feat.shape[1])
+    g = np.zeros(keep.size, np.float32)
+    loss = C.c_float(0)
+    _check(lib().madam_mlp_grad(C.byref(net), C.c_int(feat.shape[0]), feat.ctypes.data_as(_PF),
+                                target.ctypes.data_as(_PF), g.ctypes.data_as(_PF), C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(params, feat, target, epochs=1, batch=4096, lr=1e-3, seed=0):
+    """Adam on the GPU; returns (trained parameters, per-step losses)."""
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    net, p = _net(np.array(params, dtype=np.float32, copy=True), feat.shape[1])
+    N = feat.shape[0]
+    steps = (N // min(batch, N)) * epochs
+    losses = np.zeros(steps, np.float32)
+    _check(lib().madam_mlp_train(C.byref(net), C.c_longlong(N), feat.ctypes.data_as(_PF),
+                                 target.ctypes.data_as(_PF), C.c_int(epochs), C.c_int(batch),
+                                 C.c_float(lr), C.c_ulonglong(seed), losses.ctypes.data_as(_PF)))
+    return p, losses
+
+
+def pair_features(L, mu=None, sd=None):
+    L = _f64(L)
+    n = L.shape[0]
+    out = np.zeros((n * (n - 1) // 2, NFEAT), np.float32)
+    mu_p = sd_p = None
+    if mu is not None and sd is not None:
+        mu = np.ascontiguousarray(mu, dtype=np.float32)
+        sd = np.ascontiguousarray(sd, dtype=np.float32)
+        mu_p, sd_p = mu.ctypes.data_as(_PF), sd.ctypes.data_as(_PF)
+    _check(lib().madam_pair_features(C.c_int(n), L.ctypes.data_as(_PD), mu_p, sd_p,
+                                     out.ctypes.data_as(_PF)))
+    return out
+
+
+def warm_start(L, pred):
+    """(X, y, Q) for ADMM_solver from predicted pair values (madam_warm_start)."""
+    L = _f64(L)
+    n = L.shape[0]
+    pred = np.ascontiguousarray(pred, dtype=np.float32)
+    X = np.zeros((n, n), order="F")
+    Q = np.zeros((n, n), order="F")
+    y = np.zeros(n)
+    _check(lib().madam_warm_start(C.c_int(n), L.ctypes.data_as(_PD), pred.ctypes.data_as(_PF),
+                                  X.ctypes.data_as(_PD), y.ctypes.data_as(_PD),
+                                  Q.ctypes.data_as(_PD)))
+    return X, y, Q

@@ -491,7 +491,6 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
 // SDPbound (bounding.c:25-250) for k nodes in lockstep.
 int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int* ranks,
                    double* bounds) {
-  Engine& E = *g_eng;
   madam_params& P = c->params;
   int rc;
   if ((rc = ensure_slots(k))) return rc;
