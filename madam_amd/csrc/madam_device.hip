@@ -143,7 +143,9 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int max
 // gather from LDS and stream only their index arrays (aent, cact) from HBM.
 // Same operations in the same order as k_cg, so the iterates are
 // bit-identical.
-__device__ inline void bbt_lds(const NodeView& nd, const double* __restrict__ v,
+template <class Idx>
+__device__ inline void bbt_lds(const NodeView& nd, const Idx* __restrict__ aent,
+                               const Idx* __restrict__ cact, const double* __restrict__ v,
                                double* __restrict__ out, double* ps, double* w) {
   const int m = nd.m, na = nd.nact;
   for (int c = threadIdx.x; c < m; c += WG) ps[c] = nd.cscale[c] * v[c];
@@ -152,7 +154,7 @@ __device__ inline void bbt_lds(const NodeView& nd, const double* __restrict__ v,
     double acc = 0.0;
     const int e1 = nd.aptr[a + 1];
     for (int e = nd.aptr[a]; e < e1; ++e) {
-      const int en = nd.aent[e];
+      const int en = aent[e];
       const double x = ps[ent_idx(en)];
       acc = en >= 0 ? acc + x : acc - x;
     }
@@ -163,7 +165,7 @@ __device__ inline void bbt_lds(const NodeView& nd, const double* __restrict__ v,
     double acc = 0.0;
     const int e1 = nd.cptr[c + 1];
     for (int e = nd.cptr[c]; e < e1; ++e) {
-      const int ca = nd.cact[e];
+      const int ca = cact[e];
       const double x = w[ent_idx(ca)];
       acc = ca >= 0 ? acc + x : acc - x;
     }
@@ -171,10 +173,16 @@ __device__ inline void bbt_lds(const NodeView& nd, const double* __restrict__ v,
   }
 }
 
+template <class Idx>
 __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int maxit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
+  // 16-bit index streams halve the bytes each product reads (the launcher
+  // picks them only when every node has m, nact < 2^15)
+  const Idx* aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
+  const Idx* cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
+  if (!aent) return;
   const int m = nd.m;
   if (m == 0) return;
   double* ps = smem;
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
-  bbt_lds(nd, x, q, ps, w);
+  bbt_lds(nd, aent, cact, x, q, ps, w);
   double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
   for (int c = threadIdx.x; c < m; c += WG) {
     const int k = nd.cptr[c + 1] - nd.cptr[c];
@@ -203,7 +211,7 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   const double thr = rtol * rtol * bb;
   int it = 0;
   while (it < maxit && rr > thr && bb > 0.0) {
-    bbt_lds(nd, p, q, ps, w);
+    bbt_lds(nd, aent, cact, p, q, ps, w);
     double v1[1] = {0.0};
     for (int c = threadIdx.x; c < m; c += WG) v1[0] += p[c] * q[c];
     block_sums<1>(v1, red);
@@ -705,10 +713,14 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
 void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int maxit,
                size_t lds_limit, hipStream_t st) {
   const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
-  if (lds <= lds_limit)
-    k_cg_lds<<<nb, WG, lds, st>>>(nodes, rtol, maxit);
-  else
+  if (lds <= lds_limit) {
+    if (mmax < 32768 && namax < 32768)
+      k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+    else
+      k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+  } else {
     k_cg<<<nb, WG, 0, st>>>(nodes, rtol, maxit);
+  }
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

@@ -47,6 +47,7 @@ inline void set_err(const char* fmt, ...) {
 struct CutSet {
   int m = 0, nact = 0;
   std::vector<int> cptr, cent, cact, aptr, aent, apair, p2a;
+  std::vector<short> aent16, cact16;  // the same streams in 16 bits when m, nact < 2^15
   std::vector<double> cscale, y0;
 };
 
@@ -147,6 +148,12 @@ inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const 
   // cut-side pass of the BB^T product reads one index stream
   for (size_t e = 0; e < cs.cent.size(); ++e)
     if (cs.cent[e] < 0) cs.cact[e] = ~cs.cact[e];
+  cs.aent16.clear();
+  cs.cact16.clear();
+  if (m < 32768 && nact < 32768) {
+    cs.aent16.assign(cs.aent.begin(), cs.aent.end());
+    cs.cact16.assign(cs.cact.begin(), cs.cact.end());
+  }
   return MADAM_OK;
 }
 
@@ -176,6 +183,7 @@ struct DBuf {
 struct Slot {
   DBuf<double> L, X, Q, W, V, S1, S2, ev, evp, y;
   DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
+  DBuf<short> aent16, cact16;
   DBuf<double> cscale, t, s, u, rhs, r, p, q, w;
   DBuf<double> ip1, ip2, ip3, ipv, ipP;  // IPM scratch
   DBuf<float> ns32;                      // sign-iteration scratch
@@ -216,6 +224,7 @@ struct Slot {
                  na = std::max(cs.nact, 1);
     if ((rc = cptr.ensure(m + 1)) || (rc = cent.ensure(ne)) || (rc = cact.ensure(ne)) ||
         (rc = aptr.ensure(na + 1)) || (rc = aent.ensure(ne)) || (rc = apair.ensure(na)) ||
+        (rc = aent16.ensure(ne)) || (rc = cact16.ensure(ne)) ||
         (rc = p2a.ensure(nn)))
       return rc;
     for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q})
@@ -227,6 +236,8 @@ struct Slot {
                             &r, &p, &q, &w})
       b->release();
     for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a}) b->release();
+    aent16.release();
+    cact16.release();
     for (DBuf<double>* b : {&ip1, &ip2, &ip3, &ipv, &ipP, &gwval, &gwbest}) b->release();
     gwz.release();
     gwbz.release();
@@ -374,6 +385,8 @@ inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
   v.cscale = sl.cscale.p;
   v.aptr = sl.aptr.p;
   v.aent = sl.aent.p;
+  v.aent16 = cs.aent16.empty() ? nullptr : sl.aent16.p;
+  v.cact16 = cs.cact16.empty() ? nullptr : sl.cact16.p;
   v.apair = sl.apair.p;
   v.p2a = sl.p2a.p;
   v.t = sl.t.p;
@@ -399,6 +412,8 @@ inline int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {
       (rc = up(sl.cact.p, cs.cact.data(), cs.cact.size(), s)) ||
       (rc = up(sl.aptr.p, cs.aptr.data(), cs.aptr.size(), s)) ||
       (rc = up(sl.aent.p, cs.aent.data(), cs.aent.size(), s)) ||
+      (rc = up(sl.aent16.p, cs.aent16.data(), cs.aent16.size(), s)) ||
+      (rc = up(sl.cact16.p, cs.cact16.data(), cs.cact16.size(), s)) ||
       (rc = up(sl.apair.p, cs.apair.data(), cs.apair.size(), s)) ||
       (rc = up(sl.p2a.p, cs.p2a.data(), cs.p2a.size(), s)) ||
       (rc = up(sl.cscale.p, cs.cscale.data(), cs.cscale.size(), s)))

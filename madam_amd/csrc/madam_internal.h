@@ -41,6 +41,8 @@ struct NodeView {
   const double *cscale; // m
   const int *aptr;      // nact+1
   const int *aent;      // aptr[nact]: c or ~c
+  const short *aent16;  // aent and cact in 16 bits (null unless m, nact < 2^15)
+  const short *cact16;
   const int *apair;     // nact: lower linear index of the active pair
   const int *p2a;       // n*n: active-pair id of lower index lo, or -1
 

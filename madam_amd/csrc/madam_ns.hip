@@ -64,14 +64,19 @@ __device__ __forceinline__ void gemm_tile_f32(const float* __restrict__ A, const
   int i0, j0;
   tile_decode(t, i0, j0);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k0 = 0; k0 < np; k0 += 4) {
-    const int k = k0 + q;
-    const float a = A[(i0 + r) + (size_t)k * np];
-    // B symmetric: B[k][j] = B[j + k np]
-    float b = beta * B[(j0 + r) + (size_t)k * np];
-    if (j0 + r == k) b += alpha;
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  // np % 16 == 0: four k-steps per trip, all eight loads issued first
+  for (int k0 = 0; k0 < np; k0 += 16) {
+    float a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * u + q;
+      a[u] = A[(i0 + r) + (size_t)k * np];
+      // B symmetric: B[k][j] = B[j + k np]
+      b[u] = beta * B[(j0 + r) + (size_t)k * np];
+      if (j0 + r == k) b[u] += alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
   }
   // only the lower element of each (i,j)/(j,i) pair is stored, to both
   // places: the iterates stay exactly symmetric, which the transposed
@@ -107,13 +112,17 @@ __device__ __forceinline__ void gemm_tile_f64(const double* __restrict__ A, cons
   tile_decode(t, i0, j0);
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   const int ia = i0 + r, jb = j0 + r;
-#pragma unroll 4
-  for (int k0 = 0; k0 < n; k0 += 4) {
-    const int k = k0 + q;
-    const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
-    double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
-    if (jb == k && k < n) b += alpha;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * u + q;
+      a[u] = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
+      b[u] = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
+      if (jb == k && k < n) b[u] += alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {

@@ -2,9 +2,10 @@ import os
 import sys
 
 # One BLAS thread for the oracle, as the reference runs (setupenv.sh:6-8):
-# its results then do not depend on the machine's core count.
-os.environ.setdefault("MKL_NUM_THREADS", "1")
-os.environ.setdefault("OMP_NUM_THREADS", "1")
+# its results then do not depend on the machine's core count, and MKL's
+# threaded LAPACK never shares the process with torch's OpenMP runtime
+# (with both active, the oracle's dsyevr returned garbage).
+os.environ["MKL_NUM_THREADS"] = "1"
 
 import pytest
 
