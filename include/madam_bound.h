@@ -91,7 +91,8 @@ typedef struct {
     int nbit;
     double f;                 /* safe upper bound (global f of admm.c:757) */
     double lambda_max;        /* certified lambda_max used in f */
-    double *Z;                /* n x n buffer; first M columns = eigvecs*sqrt(lambda*sigma) */
+    double *Z;                /* n x n buffer; first M columns = eigvecs*sqrt(lambda*sigma)
+                                 (projection 1: a pivoted Cholesky factor, Z Z^T = X) */
     int M;
     int cg_iters;             /* total PCG iterations */
     int sweeps;               /* total Jacobi sweeps */
@@ -154,6 +155,11 @@ int madam_ns_schedule(int cap, float *coef);
 /* Certified largest eigenvalue in FP64 (the dsyevr call of admm.c:731-747):
  * returns an upper bound lam >= lambda_max(A) within |E|_F + 1e-12 |A|_F. */
 int madam_lambda_max(int n, const double *A, double *lam);
+
+/* The same bound as the mixed-precision form computes it: a Lanczos
+ * estimate certified by an FP64 Cholesky factorisation of mu I - A
+ * (lam >= lambda_max(A), within ~1e-10 |A|_F). */
+int madam_lambda_max_cert(int n, const double *A, double *lam);
 
 /* getViolated_TriangleInequalities (cutting_planes.c:48-127): the at most
  * K triangle rows with B_c(X) - 1 > thresh of largest value, in the

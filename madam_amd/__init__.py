@@ -171,6 +171,14 @@ def project_SDP_ns(W):
     return X, Q, M.value
 
 
+def lambda_max_cert(A):
+    """Certified upper bound on lambda_max(A) by Lanczos + Cholesky."""
+    A = _f64(A)
+    lam = C.c_double(0)
+    _check(lib().madam_lambda_max_cert(C.c_int(A.shape[0]), A.ctypes.data_as(_PD), C.byref(lam)))
+    return lam.value
+
+
 def ns_schedule():
     """madam_ns_schedule: (steps, 3) float32 coefficients of the sign iteration."""
     k = lib().madam_ns_schedule(0, None)

@@ -10,7 +10,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libmadam_bound.so")
 SOURCES = ["madam_device.hip", "madam_sep.hip", "madam_ipm.hip", "madam_gw.hip", "madam_ns.hip",
-           "madam_host.hip", "madam_bab.hip", "madam_mlp.hip"]
+           "madam_host.hip", "madam_bab.hip", "madam_mlp.hip",
+           "madam_cert.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -87,13 +87,18 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     if (active == 0) break;
   }
   if (o.projection == 1) {
-    // eigen-pairs of the last W for the rounding heuristic's factor Z
-    TIMED(PH_ZFAC, st,
-          madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 1, E.lds_limit, st, 2));
+    // the rounding heuristic's factor Z: pivoted Cholesky of X
+    TIMED(PH_ZFAC, st, madam::launch_zfac_chol(E.views.p, k, st));
   }
   TIMED(PH_SAFE, st, {
     madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
-    madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
+    // chosen form, nodes too large for the LDS-resident Jacobi: Lanczos +
+    // Cholesky certificate; else all eigenvalues by Jacobi, as dsyevr in
+    // the reference
+    if (o.projection == 1 && madam::jacobi_lds_bytes(nmax, false) > E.lds_limit)
+      madam::launch_safe_lc(E.views.p, k, nmax, st);
+    else
+      madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
     madam::launch_safe_bound(E.views.p, k, st);
   });
   HIP_TRY(hipGetLastError());
@@ -248,7 +253,8 @@ int madam_admm_batch(int k, madam_admm_job* jobs, const madam_opts* opts_in) {
     if (J.u && (rc = down(J.u, sl.u.p, m, st))) return rc;
     if (J.Z) {
       // Z = V_+ diag(sqrt(lambda * sigma_last)), ascending lambda, as
-      // project_SDP and admm.c:583-585 leave it.
+      // project_SDP and admm.c:583-585 leave it (the sign-iteration form:
+      // the pivoted Cholesky factor of X, in pivot order).
       evh.resize(J.n);
       Vh.resize(nn);
       if ((rc = down(Vh.data(), sl.V.p, nn, st)) || (rc = down(evh.data(), sl.evp.p, J.n, st)))
@@ -357,6 +363,22 @@ int madam_project_sdp(int n, const double* W, double* X, double* Q, double* Z, i
       const double sc = sqrt(ev[idx[q]]);
       for (int r = 0; r < n; ++r) Z[r + q * n] = Vh[r + (size_t)idx[q] * n] * sc;
     }
+  return MADAM_OK;
+}
+
+int madam_lambda_max_cert(int n, const double* A, double* lam) {
+  CutSet cs;
+  int rc = one_node(n, 0, nullptr, 0, nullptr, 0, nullptr, cs);
+  if (rc) return rc;
+  Slot& sl = g_eng->slots[0];
+  const hipStream_t st = g_eng->stream;
+  if ((rc = up(sl.W.p, A, (size_t)n * n, st))) return rc;
+  HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
+  madam::launch_safe_lc(g_eng->views.p, 1, n, st);
+  madam::launch_safe_bound(g_eng->views.p, 1, st);
+  HIP_TRY(hipGetLastError());
+  if ((rc = sync_views_down(1))) return rc;
+  *lam = g_eng->hviews[0].lam;
   return MADAM_OK;
 }
 

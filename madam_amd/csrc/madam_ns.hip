@@ -322,6 +322,94 @@ void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 64 * NS_TW, 0, st>>>(nodes, w);
 }
 
+// ---------------------------------------------------------------------------
+// The rounding heuristic's factor in the sign-iteration form, which has no
+// eigenvectors: a pivoted Cholesky X = Z Z^T of the node's primal X (PSD),
+// Z stored as V's first r columns with evp = 1 / sigma_last there and 0
+// elsewhere, so that the heuristic's V diag(sqrt(evp sigma_last)) is Z and
+// M = r.  Pivots: largest remaining diagonal, first index among equals;
+// stop when it falls below 1e-10 of the largest diagonal of X.  Any Z with
+// Z Z^T = X serves the heuristic, which rounds sign(Z v).
+__global__ __launch_bounds__(1024) void k_zfac_chol(NodeView* nodes) {
+  __shared__ double rv[16];
+  __shared__ int ri[16];
+  __shared__ double red[64];
+  NodeView& nd = nodes[blockIdx.x];
+  const int n = nd.n, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double* __restrict__ Z = nd.V;
+  double* __restrict__ d = nd.ev;  // remaining diagonal; -1 marks a pivoted row
+  const double* __restrict__ X = nd.X;
+  double mx = 0.0;
+  for (int i = tid; i < n; i += WG) {
+    d[i] = X[(size_t)i * (n + 1)];
+    nd.evp[i] = 0.0;
+    mx = fmax(mx, d[i]);
+  }
+  const double dmax0 = block_max(mx, red);
+  const double tol = 1e-10 * dmax0;
+  int r = 0;
+  for (int q = 0; q < n && dmax0 > 0.0; ++q) {
+    __syncthreads();
+    double bv = -2.0;
+    int bi = n;
+    for (int i = tid; i < n; i += WG)
+      if (d[i] > bv) {
+        bv = d[i];
+        bi = i;
+      }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      rv[w] = bv;
+      ri[w] = bi;
+    }
+    __syncthreads();
+    bv = rv[0];
+    bi = ri[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+      if (rv[k] > bv || (rv[k] == bv && ri[k] < bi)) {
+        bv = rv[k];
+        bi = ri[k];
+      }
+    if (!(bv > tol)) break;  // uniform: every thread holds the same bv
+    const int p = bi;
+    const double sp = sqrt(bv);
+    for (int i = tid; i < n; i += WG) {
+      double z = 0.0;
+      if (i == p) {
+        z = sp;
+      } else if (d[i] >= 0.0) {
+        double a = X[i + (size_t)p * n];
+        for (int t = 0; t < q; ++t) a -= Z[i + (size_t)t * n] * Z[p + (size_t)t * n];
+        z = a / sp;
+      }
+      Z[i + (size_t)q * n] = z;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += WG) {
+      if (i == p)
+        d[i] = -1.0;
+      else if (d[i] >= 0.0)
+        d[i] = fmax(0.0, d[i] - Z[i + (size_t)q * n] * Z[i + (size_t)q * n]);
+    }
+    ++r;
+  }
+  __syncthreads();
+  const double e = nd.sigma_last > 0.0 ? 1.0 / nd.sigma_last : 1.0;
+  for (int q = tid; q < r; q += WG) nd.evp[q] = e;
+  if (tid == 0) nd.M = r;
+}
+
+void launch_zfac_chol(NodeView* nodes, int nb, hipStream_t st) {
+  k_zfac_chol<<<nb, 1024, 0, st>>>(nodes);
+}
+
 // Self-test of the MFMA operand/result layouts: C32 = A (alpha I + beta B)
 // in FP32 MFMA (np x np), C64 the same in FP64 MFMA (n x n).
 __global__ __launch_bounds__(1024) void k_mfma_test(const float* A32, const float* B32, float* C32,

@@ -102,3 +102,22 @@ def test_admm_batch_equals_single():
         assert s.nbit == b.nbit
         assert s.f == b.f
         assert np.array_equal(s.X, b.X)
+
+
+@pytest.mark.parametrize("n,seed,cluster", [(3, 9, 1), (100, 10, 1), (200, 11, 6), (512, 12, 1)])
+def test_lambda_max_cert_is_an_upper_bound(n, seed, cluster):
+    """Lanczos + Cholesky certificate (the chosen form's safe bound): never
+    below lambda_max (eigh), at most 1e-8 |A|_F above it, also with a
+    multiple top eigenvalue."""
+    A = sym(n, seed, 1.0)
+    if cluster > 1:
+        w, V = np.linalg.eigh(A)
+        w[-cluster:] = w[-1]
+        A = np.asfortranarray((V * w) @ V.T)
+        A = np.asfortranarray((A + A.T) / 2)
+    lam = G.lambda_max_cert(A)
+    top = np.linalg.eigvalsh(A)[-1]
+    fro = np.linalg.norm(A)
+    print(n, "cert", lam, "eigh", top, "excess / |A|_F", (lam - top) / fro)
+    assert lam >= top - 1e-13 * fro
+    assert lam - top <= 1e-8 * fro

@@ -118,3 +118,21 @@ def test_tile_parallel_layout_is_bit_identical(graph, kgon):
     assert a.nbit == b.nbit
     assert a.f == b.f
     assert np.array_equal(a.X, b.X)
+
+
+def test_heuristic_factor_reproduces_X():
+    """In the sign-iteration form the rounding heuristic's Z is a pivoted
+    Cholesky factor of the final X: Z Z^T = X up to the part of X that is
+    not PSD (the projection's FP32 residue, eigenvalues ~1e-6 below 0),
+    which a Cholesky factor cannot carry."""
+    sol, case = admm_case(graph_L("w01_100.3"), kgon=True)
+    case["tol"] = 1e-3
+    got = G.ADMM_solver(**case, projection=1)
+    Z, X = got.Z, got.X
+    w = np.linalg.eigvalsh(X)
+    neg = max(0.0, -w.min())
+    err = np.abs(Z @ Z.T - X).max()
+    print("Z columns", Z.shape[1], "max |ZZ^T - X|", err, "most negative eigenvalue of X", -neg,
+          "max |X|", np.abs(X).max())
+    assert Z.shape[1] >= 1
+    assert err <= 20 * neg + 1e-8 * np.abs(X).max()
