@@ -281,12 +281,17 @@ def main():
         sec = eig_ms * 1e-3
         achieved = (f32 + f64) / sec / 1e12 if sec > 0 else 0.0
         t_peak = (f32 / PEAK_FP32_MFMA + f64 / PEAK_FP64_MFMA) / 1e12
+        # HBM bytes per projection of k_ns_project, measured for this workload
+        # with rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
+        # (fetch x2.00, write x1.00): 335.9 + 234.4 KiB
+        traffic = 570.3 * 1024 if args.workload == "c2-root" else None
         roofline = {"bound": "mfma", "kernel": "k_ns_project (sign-iteration projection: 39 FP32-MFMA "
                     "products + 3 FP64-MFMA products)", "achieved": achieved,
                     "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                    "frac": t_peak / sec if sec > 0 else 0.0, "traffic": None,
+                    "frac": t_peak / sec if sec > 0 else 0.0, "traffic": traffic,
                     "frac_note": "time at the FP32/FP64 MFMA peaks for these flops / measured time",
                     "launches": launches, "avg_launch_ms": eig_ms / max(1, launches),
+                    "ms_per_projection": eig_ms / max(1, rp["admm_iters"]),
                     "flops_per_projection": "84 n^3 (78 n^3 FP32 + 6 n^3 FP64)",
                     "projections": rp["admm_iters"],
                     "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
