@@ -233,15 +233,30 @@ __device__ __forceinline__ int rand31(uint64_t base, uint64_t& ctr) {
   return (int)(splitmix64(base + ctr++) >> 33);
 }
 
-__global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int njobs, SaOut* out) {
+// Jobs are laid out node by node, `per` runs each; grid (ceil(per / SA_WG),
+// nodes), so that a workgroup's runs share one X.  With XL the workgroup
+// first copies X into LDS (n^2 doubles after the permutation buffers) and
+// every run reads it there: same values, same trajectory, LDS latency.
+template <bool XL>
+__global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, SaOut* out) {
 #pragma clang fp contract(off)
   extern __shared__ unsigned short permbuf[];
-  const int r = blockIdx.x * SA_WG + threadIdx.x;
-  if (r >= njobs) return;
+  const int rr0 = blockIdx.x * SA_WG + threadIdx.x;
+  const int r = blockIdx.y * per + rr0;
+  const int first = blockIdx.y * per + blockIdx.x * SA_WG;  // a valid job of this workgroup
+  const int nst = jobs[first].nstride;
+  const double* __restrict__ X = jobs[first].X;
+  if (XL) {
+    const int nx = jobs[first].n;
+    double* Xl = (double*)(permbuf + (((size_t)SA_WG * nst + 3) & ~(size_t)3));
+    for (int f = threadIdx.x; f < nx * nx; f += SA_WG) Xl[f] = X[f];
+    __syncthreads();
+    X = Xl;
+  }
+  if (rr0 >= per) return;
   const SaJob J = jobs[r];
   const int n = J.n, k = J.k;
-  const double* __restrict__ X = J.X;
-  unsigned short* perm = permbuf + (size_t)threadIdx.x * J.nstride;
+  unsigned short* perm = permbuf + (size_t)threadIdx.x * nst;
   uint64_t base = splitmix64(J.seed ^ splitmix64((uint64_t)J.stream_id));
   uint64_t ctr = 0;
   // H = e e^T with e_a = -1 for a < neg (cutting_planes.c:242-244, 438-441)
@@ -346,10 +361,15 @@ void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipS
   k_sumx<<<(nb + 63) / 64, 64, 0, st>>>(X, n, out, nb);
 }
 
-void launch_qap_sa(const SaJob* jobs, int njobs, int nmax, SaOut* out, hipStream_t st) {
-  const int blocks = (njobs + SA_WG - 1) / SA_WG;
-  const size_t lds = (size_t)SA_WG * sa_nstride(nmax) * sizeof(unsigned short);
-  k_qap_sa<<<blocks, SA_WG, lds, st>>>(jobs, njobs, out);
+void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, SaOut* out, size_t lds_limit,
+                   hipStream_t st) {
+  const dim3 grid((per + SA_WG - 1) / SA_WG, nnodes);
+  const size_t lperm = (((size_t)SA_WG * sa_nstride(nmax) + 3) & ~(size_t)3) * sizeof(unsigned short);
+  const size_t lx = (size_t)nmax * nmax * sizeof(double);
+  if (lperm + lx <= lds_limit)
+    k_qap_sa<true><<<grid, SA_WG, lperm + lx, st>>>(jobs, per, out);
+  else
+    k_qap_sa<false><<<grid, SA_WG, lperm, st>>>(jobs, per, out);
 }
 
 }  // namespace madam
