@@ -54,33 +54,41 @@ __device__ inline void tile_decode(int t, int& i0, int& j0) {
   j0 = tj << 4;
 }
 
-// One lower 16x16 tile t of C = A (alpha I + beta B), computed by the
-// calling wave.
-__device__ __forceinline__ void gemm_tile_f32(const float* __restrict__ A, const float* __restrict__ B,
-                                     float* __restrict__ Cm, int np, float alpha, float beta,
-                                     int t) {
+// Tile products split K into four contiguous quarters, each accumulated
+// by its own MFMA chain and summed as ((q0 + q1) + q2) + q3: one wave can
+// run the four chains interleaved (independent accumulators keep the MFMA
+// pipeline full), or four waves can run one quarter each (split-K, for
+// latency when few tiles are in flight) -- with the same result bits.
+//
+// Quarter qi of a tile of C = A (alpha I + beta B): k in [qi kq, (qi+1) kq),
+// kq = n_p / 4 (a multiple of 4).  Lane layout of v_mfma_f32_16x16x4f32:
+// A operand lane l -> (row l%16, k l/16); B operand lane l -> (k l/16,
+// col l%16); result lane l -> rows 4(l/16)+v, col l%16.
+__device__ __forceinline__ f32x4 quarter_f32(const float* __restrict__ A,
+                                             const float* __restrict__ B, int np, float alpha,
+                                             float beta, int i0, int j0, int qi) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  int i0, j0;
-  tile_decode(t, i0, j0);
+  const int kq = np >> 2;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  // np % 16 == 0: four k-steps per trip, all eight loads issued first
-  for (int k0 = 0; k0 < np; k0 += 16) {
-    float a[4], b[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 4 * u + q;
-      a[u] = A[(i0 + r) + (size_t)k * np];
-      // B symmetric: B[k][j] = B[j + k np]
-      b[u] = beta * B[(j0 + r) + (size_t)k * np];
-      if (j0 + r == k) b[u] += alpha;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
+  for (int k0 = qi * kq; k0 < (qi + 1) * kq; k0 += 4) {
+    const int k = k0 + q;
+    const float a = A[(i0 + r) + (size_t)k * np];
+    // B symmetric: B[k][j] = B[j + k np]
+    float b = beta * B[(j0 + r) + (size_t)k * np];
+    if (j0 + r == k) b += alpha;
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
   }
-  // only the lower element of each (i,j)/(j,i) pair is stored, to both
-  // places: the iterates stay exactly symmetric, which the transposed
-  // operand reads rely on (an asymmetric part would grow step by step)
+  return acc;
+}
+
+// only the lower element of each (i,j)/(j,i) pair is stored, to both
+// places: the iterates stay exactly symmetric, which the transposed operand
+// reads rely on (an asymmetric part would grow step by step)
+__device__ __forceinline__ void store_tile_f32(float* __restrict__ Cm, int np, int i0, int j0,
+                                               f32x4 acc) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int i = i0 + 4 * q + v, j = j0 + r;
@@ -89,6 +97,33 @@ __device__ __forceinline__ void gemm_tile_f32(const float* __restrict__ A, const
       Cm[j + (size_t)i * np] = acc[v];
     }
   }
+}
+
+// One lower 16x16 tile t, all four quarters by the calling wave.
+__device__ __forceinline__ void gemm_tile_f32(const float* __restrict__ A, const float* __restrict__ B,
+                                              float* __restrict__ Cm, int np, float alpha,
+                                              float beta, int t) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  int i0, j0;
+  tile_decode(t, i0, j0);
+  const int kq = np >> 2;
+  f32x4 acc[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f},
+                  {0.f, 0.f, 0.f, 0.f}};
+  for (int s = 0; s < kq; s += 4) {
+    float a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = u * kq + s + q;
+      a[u] = A[(i0 + r) + (size_t)k * np];
+      b[u] = beta * B[(j0 + r) + (size_t)k * np];
+      if (j0 + r == k) b[u] += alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[u], 0, 0, 0);
+  }
+  store_tile_f32(Cm, np, i0, j0, ((acc[0] + acc[1]) + acc[2]) + acc[3]);
 }
 
 __device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
@@ -100,30 +135,31 @@ __device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const 
 }
 
 // Same in FP64 with v_mfma_f64_16x16x4f64, on n x n matrices of any n
-// (tiles past n are masked).  op: C = A (alpha I + beta B).  Operand
-// layouts as in the FP32 case; the result register v of lane l holds row
-// (l/16) + 4v, column l%16 (measured on gfx950).
-__device__ __forceinline__ void gemm_tile_f64(const double* __restrict__ A, const double* __restrict__ B,
-                                     double* __restrict__ Cm, int n, double alpha, double beta,
-                                     int t) {
+// (k and tiles past n are masked), quarters of kq = 4 ceil(n / 16).  The
+// result register v of lane l holds row (l/16) + 4v, column l%16
+// (measured on gfx950).
+__device__ __forceinline__ f64x4 quarter_f64(const double* __restrict__ A,
+                                             const double* __restrict__ B, int n, double alpha,
+                                             double beta, int i0, int j0, int qi) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  int i0, j0;
-  tile_decode(t, i0, j0);
-  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int kq = ((n + 15) >> 4) << 2;
   const int ia = i0 + r, jb = j0 + r;
-  for (int k0 = 0; k0 < n; k0 += 16) {
-    double a[4], b[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 4 * u + q;
-      a[u] = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
-      b[u] = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
-      if (jb == k && k < n) b[u] += alpha;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = qi * kq; k0 < (qi + 1) * kq; k0 += 4) {
+    const int k = k0 + q;
+    const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
+    double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
+    if (jb == k && k < n) b += alpha;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
+  return acc;
+}
+
+__device__ __forceinline__ void store_tile_f64(double* __restrict__ Cm, int n, int i0, int j0,
+                                               f64x4 acc) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int i = i0 + q + 4 * v, j = j0 + r;
@@ -132,6 +168,33 @@ __device__ __forceinline__ void gemm_tile_f64(const double* __restrict__ A, cons
       Cm[j + (size_t)i * n] = acc[v];
     }
   }
+}
+
+__device__ __forceinline__ void gemm_tile_f64(const double* __restrict__ A, const double* __restrict__ B,
+                                              double* __restrict__ Cm, int n, double alpha,
+                                              double beta, int t) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  int i0, j0;
+  tile_decode(t, i0, j0);
+  const int kq = ((n + 15) >> 4) << 2;
+  const int ia = i0 + r, jb = j0 + r;
+  f64x4 acc[4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0},
+                  {0.0, 0.0, 0.0, 0.0}};
+  for (int s = 0; s < kq; s += 4) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = u * kq + s + q;
+      a[u] = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
+      b[u] = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
+      if (jb == k && k < n) b[u] += alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[u], 0, 0, 0);
+  }
+  store_tile_f64(Cm, n, i0, j0, ((acc[0] + acc[1]) + acc[2]) + acc[3]);
 }
 
 __device__ __forceinline__ void sym_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
@@ -233,8 +296,6 @@ void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_
 // than the chip has CUs (a single node, or the stragglers of a batch); the
 // arithmetic is the one of k_ns_project, tile for tile.
 
-constexpr int NS_TW = 4;  // waves (= tiles) per workgroup
-
 // S_0 = W / |W|_F into buffer 0 of ns32 (zero padded), clear ns_fail.
 __global__ __launch_bounds__(1024) void k_ns_init(NodeView* nodes) {
   __shared__ double red[64];
@@ -256,17 +317,29 @@ __global__ __launch_bounds__(1024) void k_ns_init(NodeView* nodes) {
   if (tid == 0) nd.ns_fail = 0;
 }
 
-// C = A (alpha I + beta B) on buffers ia, ib, ic of ns32.
-__global__ __launch_bounds__(64 * NS_TW) void k_ns_gemm32(NodeView* nodes, int ia, int ib, int ic,
-                                                          float alpha, float beta) {
+// C = A (alpha I + beta B) on buffers ia, ib, ic of ns32: one workgroup of
+// four waves per tile, wave w accumulating K quarter w (split K: a quarter
+// of the latency of one wave doing the whole tile), the partial tiles summed
+// in LDS in the quarter order of gemm_tile_f32.
+__global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int ia, int ib, int ic,
+                                                   float alpha, float beta) {
+  __shared__ f32x4 part[3][64];
   NodeView& nd = nodes[blockIdx.y];
   if (nd.done) return;
   const int np = (nd.n + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x * NS_TW + (threadIdx.x >> 6);
+  const int t = blockIdx.x;
   if (t >= ntile) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int i0, j0;
+  tile_decode(t, i0, j0);
   const size_t npp = (size_t)np * np;
   float* base = nd.ns32;
-  gemm_tile_f32(base + ia * npp, base + ib * npp, base + ic * npp, np, alpha, beta, t);
+  const f32x4 acc = quarter_f32(base + ia * npp, base + ib * npp, np, alpha, beta, i0, j0, w);
+  if (w > 0) part[w - 1][lane] = acc;
+  __syncthreads();
+  if (w == 0)
+    store_tile_f32(base + ic * npp, np, i0, j0,
+                   ((acc + part[0][lane]) + part[1][lane]) + part[2][lane]);
 }
 
 // FP32 iterate (buffer is) -> FP64 n x n in S2, with the guard.
@@ -285,32 +358,39 @@ __global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int is) {
   if (bad) nd.ns_fail = 1;  // every writer stores the same value
 }
 
-// The FP64 finish, one product per launch: 0: S1 = S2^2;
+// The FP64 finish, one product per launch (split K as above): 0: S1 = S2^2;
 // 1: ip1 = S2 (1.5 I - 0.5 S1); 2: S1 = W (0.5 I + 0.5 ip1) = X_+.
-__global__ __launch_bounds__(64 * NS_TW) void k_ns_gemm64(NodeView* nodes, int which) {
+__global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int which) {
+  __shared__ f64x4 part[3][64];
   NodeView& nd = nodes[blockIdx.y];
   if (nd.done || nd.ns_fail) return;
   const int n = nd.n, nt = (n + 15) >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x * NS_TW + (threadIdx.x >> 6);
+  const int t = blockIdx.x;
   if (t >= ntile) return;
-  if (which == 0)
-    gemm_tile_f64(nd.S2, nd.S2, nd.S1, n, 0.0, 1.0, t);
-  else if (which == 1)
-    gemm_tile_f64(nd.S2, nd.S1, nd.ip1, n, 1.5, -0.5, t);
-  else
-    gemm_tile_f64(nd.W, nd.ip1, nd.S1, n, 0.5, 0.5, t);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int i0, j0;
+  tile_decode(t, i0, j0);
+  const double* A = which == 2 ? nd.W : nd.S2;
+  const double* B = which == 0 ? nd.S2 : (which == 1 ? nd.S1 : nd.ip1);
+  double* Cm = which == 1 ? nd.ip1 : nd.S1;
+  const double alpha = which == 0 ? 0.0 : (which == 1 ? 1.5 : 0.5);
+  const double beta = which == 0 ? 1.0 : (which == 1 ? -0.5 : 0.5);
+  const f64x4 acc = quarter_f64(A, B, n, alpha, beta, i0, j0, w);
+  if (w > 0) part[w - 1][lane] = acc;
+  __syncthreads();
+  if (w == 0) store_tile_f64(Cm, n, i0, j0, ((acc + part[0][lane]) + part[1][lane]) + part[2][lane]);
 }
 
 void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   static const float coef[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
   const int np = (nmax + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
-  const dim3 g32((ntile + NS_TW - 1) / NS_TW, nb);
+  const dim3 g32(ntile, nb);
   k_ns_init<<<nb, 1024, 0, st>>>(nodes);
   int sb = 0, tb = 1, ub = 2;
   for (int k = 0; k < MADAM_NS_STEPS; ++k) {
-    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, sb, sb, tb, 0.f, 1.f);               // T = S^2
-    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, tb, tb, ub, coef[k][1], coef[k][2]);  // U
-    k_ns_gemm32<<<g32, 64 * NS_TW, 0, st>>>(nodes, sb, ub, tb, coef[k][0], 1.f);         // S'
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, sb, sb, tb, 0.f, 1.f);               // T = S^2
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, tb, tb, ub, coef[k][1], coef[k][2]);  // U
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, sb, ub, tb, coef[k][0], 1.f);         // S'
     const int x = sb;
     sb = tb;
     tb = x;
@@ -318,8 +398,8 @@ void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   const int nconv = std::min(64, (nmax * nmax + 255) / 256);
   k_ns_conv<<<dim3(nconv, nb), 256, 0, st>>>(nodes, sb);
   const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
-  const dim3 g64((ntile64 + NS_TW - 1) / NS_TW, nb);
-  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 64 * NS_TW, 0, st>>>(nodes, w);
+  const dim3 g64(ntile64, nb);
+  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 256, 0, st>>>(nodes, w);
 }
 
 // ---------------------------------------------------------------------------
