@@ -88,6 +88,12 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int max
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
+  // starting guess extrapolated from the previous two solutions (as k_cg_lds)
+  for (int c = threadIdx.x; c < m; c += WG) {
+    const double tk = x[c];
+    if (nd.nbit > 0) x[c] = tk + (tk - nd.tprev[c]);
+    nd.tprev[c] = tk;
+  }
   spmv_bbt(nd, x, q, nd.cg_w);
   double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
   for (int c = threadIdx.x; c < m; c += WG) {
@@ -193,6 +199,15 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
+  // starting guess: the previous two solutions extrapolated linearly,
+  // t_k + (t_k - t_{k-1}), from the second iteration of an ADMM call on
+  // (the iterates move smoothly, so the first residual is smaller)
+  for (int c = threadIdx.x; c < m; c += WG) {
+    const double tk = x[c];
+    if (nd.nbit > 0) x[c] = tk + (tk - nd.tprev[c]);
+    nd.tprev[c] = tk;
+  }
+  __syncthreads();
   bbt_lds(nd, aent, cact, x, q, ps, w);
   double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
   for (int c = threadIdx.x; c < m; c += WG) {

@@ -47,6 +47,7 @@ struct NodeView {
   const int *p2a;       // n*n: active-pair id of lower index lo, or -1
 
   double *t, *s, *u, *rhs;             // m
+  double *tprev;                       // m: the cut solve's solution one ADMM iteration back
   double *cg_r, *cg_p, *cg_q, *cg_w;   // m, m, m, nact
 
   // per-node ADMM scalars
