@@ -13,6 +13,7 @@
 #include <stdio.h>
 
 #include <chrono>
+#include <memory>
 #include <unordered_set>
 
 #include "madam_engine.h"
@@ -488,26 +489,26 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   return MADAM_OK;
 }
 
-// SDPbound (bounding.c:25-250) for k nodes in lockstep.
-int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int* ranks,
-                   double* bounds) {
+// A run for node x in device slot `slot` (bounding.c:25-64: subproblem,
+// fixed value, sigma0, tol0).
+void init_run(madam_bab_ctx* c, Run& r, madam_node* x, int rank, int slot) {
+  r = Run();
+  r.node = x;
+  r.rank = rank;
+  r.slot = slot;
+  r.n = create_subproblem(c->N, c->SPL, x->xfixed, x->solX, r.PPL);
+  r.fixedvalue = fixed_value(c->N, c->SPL, x->xfixed, x->solX);
+  r.sigma = c->params.sigma0;
+  r.tol = c->params.tol0;
+  r.lb = c->LB;
+}
+
+// IPM warm start, first heuristic, the prune and diff tests and the first
+// triangle separation (bounding.c:65-128) for runs just started; a run
+// pruned here gets done = 2.
+int start_runs(madam_bab_ctx* c, std::vector<Run*>& all) {
   madam_params& P = c->params;
   int rc;
-  if ((rc = ensure_slots(k))) return rc;
-  std::vector<Run> runs(k);
-  std::vector<Run*> all;
-  for (int q = 0; q < k; ++q) {
-    Run& r = runs[q];
-    r.node = nodes[q];
-    r.rank = ranks ? ranks[q] : 1;
-    r.slot = q;
-    r.n = create_subproblem(c->N, c->SPL, r.node->xfixed, r.node->solX, r.PPL);
-    r.fixedvalue = fixed_value(c->N, c->SPL, r.node->xfixed, r.node->solX);
-    r.sigma = P.sigma0;
-    r.tol = P.tol0;
-    r.lb = c->LB;
-    all.push_back(&r);
-  }
   double t0 = now_s();
   if ((rc = ipm_round(c, all))) return rc;
   c->seconds_ipm += now_s() - t0;
@@ -533,80 +534,115 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
   for (Run* r : act) r->s.clear();
   if ((rc = separate_tri(c, act))) return rc;
   c->seconds_sep += now_s() - t0;
+  return MADAM_OK;
+}
 
+// One round of the cutting-plane loop (bounding.c:130-236) for the active
+// runs: an ADMM call each, the heuristic, the give-up and prune tests and
+// the separation for the runs that go on; done is set for the others.
+int cut_round(madam_bab_ctx* c, std::vector<Run*>& act) {
+  madam_params& P = c->params;
+  int rc;
+  for (Run* r : act) {
+    ++r->count;
+    r->oldf = r->f;
+  }
+  double t0 = now_s();
+  if ((rc = admm_round(c, act))) return rc;
+  c->seconds_admm += now_s() - t0;
+  std::vector<Run*> heur;
+  for (Run* r : act) {
+    r->tol *= P.scaleTol;
+    if (r->tol < P.minTol) r->tol = P.minTol;
+    r->bound = r->f + r->fixedvalue;
+    r->prune = r->bound < incumbent(c, r) + 1.0;
+    if (!r->prune) heur.push_back(r);
+  }
+  t0 = now_s();
+  if ((rc = heuristic(c, heur, false))) return rc;
+  c->seconds_heur += now_s() - t0;
+  std::vector<Run*> sep;
+  for (Run* r : act) {
+    if (!r->prune) r->prune = r->bound < incumbent(c, r) + 1.0;
+    const double gap = r->bound - incumbent(c, r);
+    if (r->count == P.triag_iter + P.pent_iter + P.hept_iter &&
+        gap - 1.0 > (r->oldf - r->f) * (P.max_outer_iter - r->count))
+      r->giveup = 1;
+    if (r->count == P.max_outer_iter && gap - 1.0 > (r->oldf - r->f) * P.extra_iter)
+      r->giveup = 1;
+    if (r->count == P.max_outer_iter + P.extra_iter) r->giveup = 1;
+    if (r->rank != 0 && r->giveup && !r->prune) {
+      P.Pent_Trials += 60;
+      P.Hepta_Trials += 50;
+    }
+    if (!r->prune && !r->giveup) sep.push_back(r);
+  }
+  t0 = now_s();
+  if (!sep.empty()) {
+    std::vector<int> triag(sep.size()), penta(sep.size());
+    for (size_t q = 0; q < sep.size(); ++q) {
+      triag[q] = (int)sep[q]->tri.size();
+      penta[q] = (int)sep[q]->pent.size();
+    }
+    if ((rc = separate_tri(c, sep))) return rc;
+    std::vector<Run*> ps, hs;
+    std::vector<int> poff, hoff;
+    for (size_t q = 0; q < sep.size(); ++q) {
+      Run* r = sep[q];
+      if (P.include_Pent && (r->count >= P.triag_iter || r->viol3 < 0.2)) {
+        ps.push_back(r);
+        poff.push_back(triag[q]);
+      }
+    }
+    if ((rc = separate_kgon(c, ps, false, poff))) return rc;
+    for (size_t q = 0; q < sep.size(); ++q) {
+      Run* r = sep[q];
+      if (P.include_Hepta && ((r->count >= P.triag_iter + P.pent_iter) ||
+                              (r->viol3 < 0.2 && (1 - r->viol5 < 0.4)))) {
+        hs.push_back(r);
+        hoff.push_back(triag[q] + penta[q]);
+      }
+    }
+    if ((rc = separate_kgon(c, hs, true, hoff))) return rc;
+  }
+  c->seconds_sep += now_s() - t0;
+  for (Run* r : act) r->done = r->prune || r->giveup || (r->nbit >= P.ADMM_itermax);
+  return fetch_fracsol(act, c->N);
+}
+
+// The bound a finished run reports (bounding.c:238-250).
+void finish_run(madam_bab_ctx* c, Run* r) {
+  if (r->done != 2) {
+    r->bound = r->f + r->fixedvalue;
+    if (r->rank == 0) c->diff = r->basic_bound - r->bound;
+  }
+  r->node->upper_bound = r->bound;
+}
+
+// SDPbound (bounding.c:25-250) for k nodes, one cutting-plane round of all
+// of them per step.
+int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int* ranks,
+                   double* bounds) {
+  int rc;
+  if ((rc = ensure_slots(k))) return rc;
+  std::vector<Run> runs(k);
+  std::vector<Run*> all;
+  for (int q = 0; q < k; ++q) {
+    init_run(c, runs[q], nodes[q], ranks ? ranks[q] : 1, q);
+    all.push_back(&runs[q]);
+  }
+  if ((rc = start_runs(c, all))) return rc;
+  std::vector<Run*> act;
+  for (Run* r : all)
+    if (!r->done) act.push_back(r);
   const double t_start = now_s();
   int round = 0;
   while (!act.empty()) {
-    for (Run* r : act) {
-      ++r->count;
-      r->oldf = r->f;
-    }
     ++round;
-    t0 = now_s();
-    if ((rc = admm_round(c, act))) return rc;
-    c->seconds_admm += now_s() - t0;
-    std::vector<Run*> heur;
-    for (Run* r : act) {
-      r->tol *= P.scaleTol;
-      if (r->tol < P.minTol) r->tol = P.minTol;
-      r->bound = r->f + r->fixedvalue;
-      r->prune = r->bound < incumbent(c, r) + 1.0;
-      if (!r->prune) heur.push_back(r);
-    }
-    t0 = now_s();
-    if ((rc = heuristic(c, heur, false))) return rc;
-    c->seconds_heur += now_s() - t0;
-    std::vector<Run*> sep;
-    for (Run* r : act) {
-      if (!r->prune) r->prune = r->bound < incumbent(c, r) + 1.0;
-      const double gap = r->bound - incumbent(c, r);
-      if (r->count == P.triag_iter + P.pent_iter + P.hept_iter &&
-          gap - 1.0 > (r->oldf - r->f) * (P.max_outer_iter - r->count))
-        r->giveup = 1;
-      if (r->count == P.max_outer_iter && gap - 1.0 > (r->oldf - r->f) * P.extra_iter)
-        r->giveup = 1;
-      if (r->count == P.max_outer_iter + P.extra_iter) r->giveup = 1;
-      if (r->rank != 0 && r->giveup && !r->prune) {
-        P.Pent_Trials += 60;
-        P.Hepta_Trials += 50;
-      }
-      if (!r->prune && !r->giveup) sep.push_back(r);
-    }
-    t0 = now_s();
-    if (!sep.empty()) {
-      std::vector<int> triag(sep.size()), penta(sep.size());
-      for (size_t q = 0; q < sep.size(); ++q) {
-        triag[q] = (int)sep[q]->tri.size();
-        penta[q] = (int)sep[q]->pent.size();
-      }
-      if ((rc = separate_tri(c, sep))) return rc;
-      std::vector<Run*> ps, hs;
-      std::vector<int> poff, hoff;
-      for (size_t q = 0; q < sep.size(); ++q) {
-        Run* r = sep[q];
-        if (P.include_Pent && (r->count >= P.triag_iter || r->viol3 < 0.2)) {
-          ps.push_back(r);
-          poff.push_back(triag[q]);
-        }
-      }
-      if ((rc = separate_kgon(c, ps, false, poff))) return rc;
-      for (size_t q = 0; q < sep.size(); ++q) {
-        Run* r = sep[q];
-        if (P.include_Hepta && ((r->count >= P.triag_iter + P.pent_iter) ||
-                                (r->viol3 < 0.2 && (1 - r->viol5 < 0.4)))) {
-          hs.push_back(r);
-          hoff.push_back(triag[q] + penta[q]);
-        }
-      }
-      if ((rc = separate_kgon(c, hs, true, hoff))) return rc;
-    }
-    c->seconds_sep += now_s() - t0;
+    if ((rc = cut_round(c, act))) return rc;
     std::vector<Run*> next;
-    for (Run* r : act) {
-      r->done = r->prune || r->giveup || (r->nbit >= P.ADMM_itermax);
+    for (Run* r : act)
       if (!r->done) next.push_back(r);
-    }
-    if ((rc = fetch_fracsol(act, c->N))) return rc;
     if (c->verbose > 0) {
       int nbmax = 0;
       size_t mmax = 0;
@@ -621,12 +657,8 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
     act.swap(next);
   }
   for (Run* r : all) {
-    if (r->done != 2) {
-      r->bound = r->f + r->fixedvalue;
-      if (r->rank == 0) c->diff = r->basic_bound - r->bound;
-    }
+    finish_run(c, r);
     bounds[r->slot] = r->bound;
-    r->node->upper_bound = r->bound;
   }
   c->nodes += k;
   return MADAM_OK;
@@ -759,26 +791,55 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
     h.push(child_of(c, root, ic, 1));
   }
   delete root;
-  while (!h.d.empty()) {
-    if (max_nodes > 0 && c->nodes >= max_nodes) break;
-    std::vector<madam_node*> cur;
-    while (!h.d.empty() && (int)cur.size() < batch) {
+  // Node pipeline: up to `batch` nodes in flight, one cutting-plane round of
+  // all of them per step; a node that finishes is branched at once and its
+  // slot refilled from the heap, so the batch stays full while the heap
+  // has nodes (in place of bounding `batch` nodes to the last straggler).
+  std::vector<std::unique_ptr<Run>> pool;
+  std::vector<int> free_slots;
+  for (int q = batch - 1; q >= 0; --q) free_slots.push_back(q);
+  if ((rc = ensure_slots(batch))) {
+    for (auto* x : h.d) delete x;
+    return rc;
+  }
+  long long started = c->nodes;
+  const double t_start = now_s();
+  double t_print = t_start;
+  auto fail = [&](int code) {
+    for (auto& r : pool) delete r->node;
+    for (auto* x : h.d) delete x;
+    return code;
+  };
+  for (;;) {
+    std::vector<Run*> fresh;
+    while ((int)pool.size() < batch && !h.d.empty() && (max_nodes <= 0 || started < max_nodes)) {
       madam_node* x = h.pop();
-      if (c->LB + 1.0 < x->upper_bound)
-        cur.push_back(x);
-      else
+      if (!(c->LB + 1.0 < x->upper_bound)) {
         delete x;  // the parent's bound already prunes it
+        continue;
+      }
+      pool.emplace_back(new Run());
+      init_run(c, *pool.back(), x, 1, free_slots.back());
+      free_slots.pop_back();
+      fresh.push_back(pool.back().get());
+      ++started;
     }
-    if (cur.empty()) continue;
-    std::vector<int> ranks(cur.size(), 1);
-    std::vector<double> bs(cur.size());
-    if ((rc = sdpbound_batch(c, (int)cur.size(), cur.data(), ranks.data(), bs.data()))) {
-      for (auto* x : cur) delete x;
-      for (auto* x : h.d) delete x;
-      return rc;
-    }
-    for (size_t q = 0; q < cur.size(); ++q) {
-      madam_node* x = cur[q];
+    if (pool.empty()) break;
+    if (!fresh.empty() && (rc = start_runs(c, fresh))) return fail(rc);
+    std::vector<Run*> act;
+    for (auto& r : pool)
+      if (!r->done) act.push_back(r.get());
+    if (!act.empty() && (rc = cut_round(c, act))) return fail(rc);
+    // finished nodes: bound, branch, free the slot
+    for (size_t q = 0; q < pool.size();) {
+      Run* r = pool[q].get();
+      if (!r->done) {
+        ++q;
+        continue;
+      }
+      finish_run(c, r);
+      madam_node* x = r->node;
+      c->nodes += 1;
       if (c->LB + 1.0 < x->upper_bound) {
         const int ic = branch_var(c, x);
         if (ic >= 0) {
@@ -787,10 +848,14 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
         }
       }
       delete x;
+      free_slots.push_back(r->slot);
+      pool[q] = std::move(pool.back());
+      pool.pop_back();
     }
-    if (c->verbose > 0) {
-      fprintf(stderr, "bab: %lld nodes bounded, %zu open, incumbent %.0f\n", c->nodes, h.d.size(),
-              c->LB);
+    if (c->verbose > 0 && (now_s() - t_print > 2.0 || pool.empty())) {
+      t_print = now_s();
+      fprintf(stderr, "bab: %lld nodes bounded, %zu in flight, %zu open, incumbent %.0f, %.2f s\n",
+              c->nodes, pool.size(), h.d.size(), c->LB, t_print - t_start);
       fflush(stderr);
     }
   }

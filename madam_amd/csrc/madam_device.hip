@@ -88,10 +88,14 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int max
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
-  // starting guess extrapolated from the previous two solutions (as k_cg_lds)
+  // starting guess extrapolated from the previous solutions (as k_cg_lds)
   for (int c = threadIdx.x; c < m; c += WG) {
-    const double tk = x[c];
-    if (nd.nbit > 0) x[c] = tk + (tk - nd.tprev[c]);
+    const double tk = x[c], t1 = nd.tprev[c];
+    if (nd.nbit > 1)
+      x[c] = 3.0 * (tk - t1) + nd.tprev2[c];
+    else if (nd.nbit > 0)
+      x[c] = tk + (tk - t1);
+    nd.tprev2[c] = t1;
     nd.tprev[c] = tk;
   }
   spmv_bbt(nd, x, q, nd.cg_w);
@@ -199,12 +203,17 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
-  // starting guess: the previous two solutions extrapolated linearly,
-  // t_k + (t_k - t_{k-1}), from the second iteration of an ADMM call on
-  // (the iterates move smoothly, so the first residual is smaller)
+  // starting guess: the previous solutions extrapolated -- quadratically,
+  // 3 t_k - 3 t_{k-1} + t_{k-2}, from the third iteration of an ADMM call
+  // on, linearly at the second (the iterates move smoothly, so the first
+  // residual is smaller)
   for (int c = threadIdx.x; c < m; c += WG) {
-    const double tk = x[c];
-    if (nd.nbit > 0) x[c] = tk + (tk - nd.tprev[c]);
+    const double tk = x[c], t1 = nd.tprev[c];
+    if (nd.nbit > 1)
+      x[c] = 3.0 * (tk - t1) + nd.tprev2[c];
+    else if (nd.nbit > 0)
+      x[c] = tk + (tk - t1);
+    nd.tprev2[c] = t1;
     nd.tprev[c] = tk;
   }
   __syncthreads();

@@ -184,7 +184,7 @@ struct Slot {
   DBuf<double> L, X, Q, W, V, S1, S2, ev, evp, y;
   DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
   DBuf<short> aent16, cact16;
-  DBuf<double> cscale, t, s, u, rhs, r, p, q, w, tprev;
+  DBuf<double> cscale, t, s, u, rhs, r, p, q, w, tprev, tprev2;
   DBuf<double> ip1, ip2, ip3, ipv, ipP;  // IPM scratch
   DBuf<float> ns32;                      // sign-iteration scratch
 
@@ -227,13 +227,13 @@ struct Slot {
         (rc = aent16.ensure(ne)) || (rc = cact16.ensure(ne)) ||
         (rc = p2a.ensure(nn)))
       return rc;
-    for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q, &tprev})
+    for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q, &tprev, &tprev2})
       if ((rc = b->ensure(m))) return rc;
     return w.ensure(na);
   }
   void release() {
     for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2, &ev, &evp, &y, &cscale, &t, &s, &u, &rhs,
-                            &r, &p, &q, &w, &tprev})
+                            &r, &p, &q, &w, &tprev, &tprev2})
       b->release();
     for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a}) b->release();
     aent16.release();
@@ -394,6 +394,7 @@ inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
   v.u = sl.u.p;
   v.rhs = sl.rhs.p;
   v.tprev = sl.tprev.p;
+  v.tprev2 = sl.tprev2.p;
   v.cg_r = sl.r.p;
   v.cg_p = sl.p.p;
   v.cg_q = sl.q.p;

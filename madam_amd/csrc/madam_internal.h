@@ -47,7 +47,7 @@ struct NodeView {
   const int *p2a;       // n*n: active-pair id of lower index lo, or -1
 
   double *t, *s, *u, *rhs;             // m
-  double *tprev;                       // m: the cut solve's solution one ADMM iteration back
+  double *tprev, *tprev2;              // m: the cut solve's solutions one and two iterations back
   double *cg_r, *cg_p, *cg_q, *cg_w;   // m, m, m, nact
 
   // per-node ADMM scalars
