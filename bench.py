@@ -228,7 +228,45 @@ def wl_c4(args):
     return cfg, step, cpu
 
 
-WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4}
+def wl_bab150(args):
+    """Branch and bound of pw05_150.0 (rudy, n=150; the reference recorded
+    optimum 17852, root bound 18083.50, 66941 nodes in 576 s on 240 cores).
+    Chosen: node pipeline of 256 slots; a step bounds args.bab_nodes nodes
+    (0: the whole tree, to the proven optimum).  Plain: one node at a time,
+    FP64 Jacobi, on the first 40 nodes; CPU: the oracle on the first 40."""
+    import madam_amd as G
+    from madam_amd import instances as I
+    path = os.path.join(ROOT, "tests", "golden", "rudy150", "pw05_150.0")
+    SPL = I.objective_matrix(*I.read_rudy(path))
+
+    def step(form, sample=None):
+        if form == "chosen":
+            B = G.Bounder(SPL, G.default_params(), batch=256, verbose=1, **CHOSEN)
+            opt = B.bab(max_nodes=args.bab_nodes)
+        else:
+            B = G.Bounder(SPL, G.default_params(), batch=1, **PLAIN)
+            opt = B.bab(max_nodes=40)
+        return {"bound": float(opt), "bounds": np.array([opt]), "admm_iters": int(B.ctx.admm_iters),
+                "nodes": int(B.ctx.nodes), "n3_iters": float(B.ctx.admm_iters) * 140 ** 3,
+                "open": int(B.ctx.open_nodes)}
+
+    def cpu(seconds):
+        from oracle import oracle_py as O
+        C.CDLL(None).srand(2020)
+        t0 = time.perf_counter()
+        s = O.Solver(SPL, O.default_params())
+        s.bab(40)
+        dt = time.perf_counter() - t0
+        return s.s.nodes / dt, f"oracle B&B of pw05_150.0, first {s.s.nodes} nodes in {dt:.1f} s", None
+
+    cfg = {"workload": "branch and bound of pw05_150.0 (n=150, rudy), node pipeline of 256, "
+                       + ("to the proven optimum" if args.bab_nodes <= 0 else f"first {args.bab_nodes} nodes"),
+           "n": 150, "nodes_per_step": "B&B"}
+    return cfg, step, cpu
+
+
+WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4,
+             "bab150": wl_bab150}
 
 
 def main():
@@ -242,6 +280,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--c4-train", type=int, default=2000, help="c4-learned: training graphs")
+    ap.add_argument("--bab-nodes", type=int, default=20000,
+                    help="bab150: nodes per step (0: solve to the proven optimum)")
     ap.add_argument("--plain-nodes", type=int, default=4,
                     help="c3-batch: nodes bounded one at a time by the plain form")
     args = ap.parse_args()
@@ -327,7 +367,7 @@ def main():
             "dtype": "f32-mfma+f64", "data": "synthetic", "config": cfg, "roofline": roofline,
             "plain_fp64_gpu": plain, "cpu_baseline": cpub,
             "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
-    if args.workload == "c1-bab":
+    if args.workload == "c1-bab" or (args.workload == "bab150" and r.get("open") == 0):
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]
     if args.workload == "c4-learned" and plain is not None:
