@@ -233,7 +233,11 @@ def wl_bab150(args):
     optimum 17852, root bound 18083.50, 66941 nodes in 576 s on 240 cores).
     Chosen: node pipeline of 256 slots; a step bounds args.bab_nodes nodes
     (0: the whole tree, to the proven optimum).  Plain: one node at a time,
-    FP64 Jacobi, on the first 40 nodes; CPU: the oracle on the first 40."""
+    FP64 Jacobi, on the first 200 nodes, and beside it the chosen form one
+    node at a time on the same 200 nodes (the projection and cut-solve gain
+    without batching); CPU: the oracle on the first 40.  Early nodes are
+    mostly branched after the IPM bound (the diff test), deep ones run the
+    whole cutting-plane loop, so nodes/s depends on how far the search got."""
     import madam_amd as G
     from madam_amd import instances as I
     path = os.path.join(ROOT, "tests", "golden", "rudy150", "pw05_150.0")
@@ -244,11 +248,21 @@ def wl_bab150(args):
             B = G.Bounder(SPL, G.default_params(), batch=256, verbose=1, **CHOSEN)
             opt = B.bab(max_nodes=args.bab_nodes)
         else:
+            B1 = G.Bounder(SPL, G.default_params(), batch=1, **CHOSEN)
+            t1 = time.perf_counter()
+            B1.bab(max_nodes=200)
+            t1 = time.perf_counter() - t1
+            extra = {"chosen_unbatched_same_nodes": {
+                "value": int(B1.ctx.nodes) / t1, "nodes": int(B1.ctx.nodes),
+                "admm_calls": int(B1.ctx.admm_calls), "incumbent": float(B1.ctx.LB)}}
             B = G.Bounder(SPL, G.default_params(), batch=1, **PLAIN)
-            opt = B.bab(max_nodes=40)
-        return {"bound": float(opt), "bounds": np.array([opt]), "admm_iters": int(B.ctx.admm_iters),
-                "nodes": int(B.ctx.nodes), "n3_iters": float(B.ctx.admm_iters) * 140 ** 3,
-                "open": int(B.ctx.open_nodes)}
+            opt = B.bab(max_nodes=200)
+        r = {"bound": float(opt), "bounds": np.array([opt]), "admm_iters": int(B.ctx.admm_iters),
+             "nodes": int(B.ctx.nodes), "n3_iters": float(B.ctx.admm_iters) * 140 ** 3,
+             "open": int(B.ctx.open_nodes), "admm_calls": int(B.ctx.admm_calls)}
+        if form != "chosen":
+            r["extra"] = extra
+        return r
 
     def cpu(seconds):
         from oracle import oracle_py as O
@@ -349,6 +363,9 @@ def main():
                  "chosen_over_plain": value / (pr["nodes"] / tp)}
         diff = np.abs(r["bounds"][:nb] - pr["bounds"]) / np.abs(pr["bounds"])
         plain["rel_bound_diff_max"] = float(diff.max())
+        plain.update(pr.get("extra", {}))
+        if "admm_calls" in pr:
+            plain["admm_calls"] = pr["admm_calls"]
         if args.workload == "c1-bab":
             plain["time_to_optimum_s"] = tp
 
@@ -367,6 +384,8 @@ def main():
             "dtype": "f32-mfma+f64", "data": "synthetic", "config": cfg, "roofline": roofline,
             "plain_fp64_gpu": plain, "cpu_baseline": cpub,
             "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
+    if "admm_calls" in r:
+        line["chosen"]["admm_calls"] = r["admm_calls"]
     if args.workload == "c1-bab" or (args.workload == "bab150" and r.get("open") == 0):
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]

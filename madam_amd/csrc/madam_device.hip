@@ -268,6 +268,152 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   }
 }
 
+// The same PCG once more, with every cut-side vector in registers: thread
+// tid owns cuts tid + j WG (j < CPT) and keeps x, r, p and the diagonal
+// preconditioner of each in registers for the whole solve, so an iteration
+// touches HBM only for the two index streams (cptr/cact of its own cuts and
+// aptr/aent of the pair pass).  ps and w stay in LDS (the pair pass gathers
+// them across threads).  Same operations in the same order as k_cg (each
+// thread's partial sums run over its cuts in increasing order), so the
+// iterates are bit-identical; used when m <= CPT WG (CPT 2 or 4: eight
+// cuts per thread would spill registers).
+template <int CPT, class Idx>
+__global__ __launch_bounds__(WG) void k_cg_reg(NodeView* nodes, double rtol, int maxit) {
+  extern __shared__ double smem[];
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done) return;
+  const Idx* __restrict__ aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
+  const Idx* __restrict__ cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
+  if (!aent) return;
+  const int m = nd.m, na = nd.nact, tid = threadIdx.x;
+  if (m == 0) return;
+  double* ps = smem;
+  double* w = smem + m;
+  double* red = w + na;
+  double x[CPT], r[CPT], p[CPT], dinv[CPT], sc[CPT];
+  int e0[CPT], e1[CPT];
+  double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = tid + j * WG;
+    x[j] = r[j] = p[j] = 0.0;
+    dinv[j] = sc[j] = 0.0;
+    e0[j] = e1[j] = 0;
+    if (c < m) {
+      // starting guess extrapolated from the previous solutions (as k_cg)
+      const double tk = nd.t[c], t1 = nd.tprev[c];
+      double xc = tk;
+      if (nd.nbit > 1)
+        xc = 3.0 * (tk - t1) + nd.tprev2[c];
+      else if (nd.nbit > 0)
+        xc = tk + (tk - t1);
+      nd.tprev2[c] = t1;
+      nd.tprev[c] = tk;
+      x[j] = xc;
+      e0[j] = nd.cptr[c];
+      e1[j] = nd.cptr[c + 1];
+      sc[j] = nd.cscale[c];
+      dinv[j] = 1.0 / (1.0 + 0.5 * (e1[j] - e0[j]) * sc[j] * sc[j]);
+      ps[c] = sc[j] * xc;
+    }
+  }
+  __syncthreads();
+  // (BB^T + I) v for the cut-side vector v whose scaled copy is in ps:
+  // pair pass into w, then each thread's own cuts (out[j] = v[j] + ...)
+  auto pair_pass = [&]() {
+    for (int a = tid; a < na; a += WG) {
+      double acc = 0.0;
+      const int f1 = nd.aptr[a + 1];
+      for (int e = nd.aptr[a]; e < f1; ++e) {
+        const int en = aent[e];
+        const double v = ps[ent_idx(en)];
+        acc = en >= 0 ? acc + v : acc - v;
+      }
+      w[a] = acc;
+    }
+    __syncthreads();
+  };
+  auto cut_acc = [&](int j) {
+    double acc = 0.0;
+    for (int e = e0[j]; e < e1[j]; ++e) {
+      const int ca = cact[e];
+      const double v = w[ent_idx(ca)];
+      acc = ca >= 0 ? acc + v : acc - v;
+    }
+    return acc;
+  };
+  pair_pass();
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = tid + j * WG;
+    if (c < m) {
+      const double q = x[j] + 0.5 * sc[j] * cut_acc(j);
+      const double b = nd.rhs[c];
+      const double rc = b - q;
+      r[j] = rc;
+      p[j] = rc * dinv[j];
+      v3[0] += rc * rc * dinv[j];
+      v3[1] += rc * rc;
+      v3[2] += b * b;
+    }
+  }
+  block_sums<3>(v3, red);
+  double rz = v3[0], rr = v3[1];
+  const double bb = v3[2];
+  const double thr = rtol * rtol * bb;
+  int it = 0;
+  while (it < maxit && rr > thr && bb > 0.0) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int c = tid + j * WG;
+      if (c < m) ps[c] = sc[j] * p[j];
+    }
+    __syncthreads();
+    pair_pass();
+    double q[CPT];
+    double v1[1] = {0.0};
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int c = tid + j * WG;
+      q[j] = 0.0;
+      if (c < m) {
+        q[j] = p[j] + 0.5 * sc[j] * cut_acc(j);
+        v1[0] += p[j] * q[j];
+      }
+    }
+    block_sums<1>(v1, red);
+    const double alpha = rz / v1[0];
+    double v2[2] = {0, 0};
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int c = tid + j * WG;
+      if (c < m) {
+        x[j] += alpha * p[j];
+        const double rc = r[j] - alpha * q[j];
+        r[j] = rc;
+        v2[0] += rc * rc * dinv[j];
+        v2[1] += rc * rc;
+      }
+    }
+    block_sums<2>(v2, red);
+    const double beta = v2[0] / rz;
+    rz = v2[0];
+    rr = v2[1];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) p[j] = r[j] * dinv[j] + beta * p[j];
+    ++it;
+  }
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = tid + j * WG;
+    if (c < m) nd.t[c] = x[j];
+  }
+  if (tid == 0) {
+    nd.cg_iters = it;
+    nd.cg_total += it;
+  }
+}
+
 // Cut duals (admm.c:557-560, 589-591) and the matrix to project
 // W = L + X/sigma - Diag(y) - B^T(t) (admm.c:564-568); with use_s the safe
 // bound matrix L - Diag(y) - B^T(s) (admm.c:727-729) instead.  Lower
@@ -738,7 +884,18 @@ void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int ma
                size_t lds_limit, hipStream_t st) {
   const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
   if (lds <= lds_limit) {
-    if (mmax < 32768 && namax < 32768)
+    const bool i16 = mmax < 32768 && namax < 32768;
+    if (mmax <= 2 * WG) {
+      if (i16)
+        k_cg_reg<2, short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+      else
+        k_cg_reg<2, int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+    } else if (mmax <= 4 * WG) {
+      if (i16)
+        k_cg_reg<4, short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+      else
+        k_cg_reg<4, int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+    } else if (i16)
       k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
     else
       k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
