@@ -268,36 +268,117 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   }
 }
 
-// The same PCG once more, with every cut-side vector in registers: thread
-// tid owns cuts tid + j WG (j < CPT) and keeps x, r, p and the diagonal
-// preconditioner of each in registers for the whole solve, so an iteration
-// touches HBM only for the two index streams (cptr/cact of its own cuts and
-// aptr/aent of the pair pass).  ps and w stay in LDS (the pair pass gathers
-// them across threads).  Same operations in the same order as k_cg (each
-// thread's partial sums run over its cuts in increasing order), so the
-// iterates are bit-identical; used when m <= CPT WG (CPT 2 or 4: eight
-// cuts per thread would spill registers).
-template <int CPT, class Idx>
-__global__ __launch_bounds__(WG) void k_cg_reg(NodeView* nodes, double rtol, int maxit) {
+// sum of +-v[idx] over the signed entries ent[e0..e1) in entry order,
+// four loads in flight at a time (the sum itself stays sequential)
+template <class Idx>
+__device__ __forceinline__ double signed_gather(const Idx* __restrict__ ent,
+                                                const double* __restrict__ v, int e0, int e1) {
+  double acc = 0.0;
+  int e = e0;
+  for (; e + 4 <= e1; e += 4) {
+    const int k0 = ent[e], k1 = ent[e + 1], k2 = ent[e + 2], k3 = ent[e + 3];
+    const double v0 = v[ent_idx(k0)], v1 = v[ent_idx(k1)], v2 = v[ent_idx(k2)],
+                 v3 = v[ent_idx(k3)];
+    acc = k0 >= 0 ? acc + v0 : acc - v0;
+    acc = k1 >= 0 ? acc + v1 : acc - v1;
+    acc = k2 >= 0 ? acc + v2 : acc - v2;
+    acc = k3 >= 0 ? acc + v3 : acc - v3;
+  }
+  for (; e < e1; ++e) {
+    const int k = ent[e];
+    const double x = v[ent_idx(k)];
+    acc = k >= 0 ? acc + x : acc - x;
+  }
+  return acc;
+}
+
+// The cut solve in the Chronopoulos-Gear form of PCG (one reduction per
+// iteration instead of two: gamma = (r, u), delta = (A u, u) and |r|^2 are
+// summed together, and s = A p is carried by recurrence), with every
+// cut-side vector in registers: each thread owns up to CPT cuts and keeps
+// x, r, p, s, the diagonal preconditioner and the cut scale of each for the
+// whole solve.  ps (scale * u, gathered by the pair pass) and
+// w (pair sums) are in LDS; with STAGE the index streams (aptr, aent, cact)
+// are copied to LDS once per solve, so the iterations read no HBM at all.
+// Per iteration: one product (two barriers) and one reduction (one
+// barrier).  Same linear system, start and stopping rule as k_cg; the
+// iterates agree with it to rounding.  Used when m <= CPT WG (CPT 2 or 4:
+// more cuts per thread would spill registers).
+//   r0 = b - A x0, u0 = D r0, w0 = A u0, alpha0 = (r0,u0) / (w0,u0)
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = D r,
+//   w = A u, gamma' = (r,u), delta = (w,u), beta = gamma' / gamma,
+//   alpha = gamma' / (delta - beta gamma' / alpha)
+template <int CPT, class Idx, bool STAGE>
+__global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, int maxit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
-  const Idx* __restrict__ aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
-  const Idx* __restrict__ cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
+  const Idx* aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
+  const Idx* cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
   if (!aent) return;
-  const int m = nd.m, na = nd.nact, tid = threadIdx.x;
+  const int m = nd.m, na = nd.nact, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (m == 0) return;
   double* ps = smem;
   double* w = smem + m;
   double* red = w + na;
-  double x[CPT], r[CPT], p[CPT], dinv[CPT], sc[CPT];
+  const int* aptr = nd.aptr;
+  if (STAGE) {
+    const int ne = nd.ne;
+    int* sapt = (int*)(red + 64);
+    Idx* sae = (Idx*)(sapt + na + 1);
+    Idx* sca = sae + ne;
+    for (int i = tid; i <= na; i += WG) sapt[i] = nd.aptr[i];
+    for (int i = tid; i < ne; i += WG) {
+      sae[i] = aent[i];
+      sca[i] = cact[i];
+    }
+    aptr = sapt;
+    aent = sae;
+    cact = sca;  // visible after the barrier that follows the start below
+  }
+  // reduction with one barrier: every call below is separated from the
+  // previous one by the two barriers of a product, so no thread can still
+  // be reading red when another writes it
+  auto reduce = [&](auto& v) {
+    constexpr int NV = sizeof(v) / sizeof(double);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) red[q * 16 + wv] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double a = 0.0;
+      for (int i = 0; i < WG / 64; ++i) a += red[q * 16 + i];
+      v[q] = a;
+    }
+  };
+  // pair pass w = sum of the signed ps over the cuts of each active pair,
+  // then the cut-side sums of w for the calling thread's cuts
+  auto pair_pass = [&]() {
+    for (int a = tid; a < na; a += WG) w[a] = signed_gather(aent, ps, aptr[a], aptr[a + 1]);
+    __syncthreads();
+  };
+  // cuts come ordered triangles (3 pairs), pentagonal (10), heptagonal
+  // (21): thread tid owns cut q and cut m-1-q for q = tid + i WG, so every
+  // thread gets a like share of pair entries
+  const int half = (m + 1) >> 1;
+  auto cut_of = [&](int j) {
+    const int q = tid + (j >> 1) * WG;
+    if (q >= half) return m;  // none
+    if (j & 1) return m - 1 - q > q ? m - 1 - q : m;
+    return q;
+  };
+  // u = D r is recomputed where needed rather than kept (registers)
+  double x[CPT], r[CPT], p[CPT], s[CPT], dinv[CPT], sc[CPT];
   int e0[CPT], e1[CPT];
-  double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
+  auto cut_acc = [&](int j) { return signed_gather(cact, w, e0[j], e1[j]); };
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
-    const int c = tid + j * WG;
-    x[j] = r[j] = p[j] = 0.0;
-    dinv[j] = sc[j] = 0.0;
+    const int c = cut_of(j);
+    x[j] = r[j] = p[j] = s[j] = dinv[j] = sc[j] = 0.0;
     e0[j] = e1[j] = 0;
     if (c < m) {
       // starting guess extrapolated from the previous solutions (as k_cg)
@@ -318,94 +399,88 @@ __global__ __launch_bounds__(WG) void k_cg_reg(NodeView* nodes, double rtol, int
     }
   }
   __syncthreads();
-  // (BB^T + I) v for the cut-side vector v whose scaled copy is in ps:
-  // pair pass into w, then each thread's own cuts (out[j] = v[j] + ...)
-  auto pair_pass = [&]() {
-    for (int a = tid; a < na; a += WG) {
-      double acc = 0.0;
-      const int f1 = nd.aptr[a + 1];
-      for (int e = nd.aptr[a]; e < f1; ++e) {
-        const int en = aent[e];
-        const double v = ps[ent_idx(en)];
-        acc = en >= 0 ? acc + v : acc - v;
-      }
-      w[a] = acc;
-    }
-    __syncthreads();
-  };
-  auto cut_acc = [&](int j) {
-    double acc = 0.0;
-    for (int e = e0[j]; e < e1[j]; ++e) {
-      const int ca = cact[e];
-      const double v = w[ent_idx(ca)];
-      acc = ca >= 0 ? acc + v : acc - v;
-    }
-    return acc;
-  };
   pair_pass();
+  double v3[3] = {0, 0, 0};  // (r,u), (r,r), (b,b)
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
-    const int c = tid + j * WG;
+    const int c = cut_of(j);
     if (c < m) {
-      const double q = x[j] + 0.5 * sc[j] * cut_acc(j);
       const double b = nd.rhs[c];
-      const double rc = b - q;
+      const double rc = b - (x[j] + 0.5 * sc[j] * cut_acc(j));
       r[j] = rc;
-      p[j] = rc * dinv[j];
-      v3[0] += rc * rc * dinv[j];
+      const double uc = rc * dinv[j];
+      v3[0] += rc * uc;
       v3[1] += rc * rc;
       v3[2] += b * b;
     }
   }
-  block_sums<3>(v3, red);
-  double rz = v3[0], rr = v3[1];
+  reduce(v3);
+  double gamma = v3[0], rr = v3[1];
   const double bb = v3[2];
   const double thr = rtol * rtol * bb;
+  double alpha = 0.0, beta = 0.0;
   int it = 0;
-  while (it < maxit && rr > thr && bb > 0.0) {
+  bool run = it < maxit && rr > thr && bb > 0.0;
+  double wl[CPT];
+  if (run) {
+    // w0 = A u0, alpha0 = (r0,u0) / (w0,u0)
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int c = tid + j * WG;
-      if (c < m) ps[c] = sc[j] * p[j];
+      const int c = cut_of(j);
+      if (c < m) ps[c] = sc[j] * (r[j] * dinv[j]);
     }
     __syncthreads();
     pair_pass();
-    double q[CPT];
     double v1[1] = {0.0};
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int c = tid + j * WG;
-      q[j] = 0.0;
+      const int c = cut_of(j);
+      wl[j] = 0.0;
       if (c < m) {
-        q[j] = p[j] + 0.5 * sc[j] * cut_acc(j);
-        v1[0] += p[j] * q[j];
+        const double uc = r[j] * dinv[j];
+        wl[j] = uc + 0.5 * sc[j] * cut_acc(j);
+        v1[0] += wl[j] * uc;
       }
     }
-    block_sums<1>(v1, red);
-    const double alpha = rz / v1[0];
-    double v2[2] = {0, 0};
+    reduce(v1);
+    alpha = gamma / v1[0];
+  }
+  while (run) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int c = tid + j * WG;
+      const int c = cut_of(j);
+      p[j] = r[j] * dinv[j] + beta * p[j];
+      s[j] = wl[j] + beta * s[j];
+      x[j] += alpha * p[j];
+      r[j] -= alpha * s[j];
+      if (c < m) ps[c] = sc[j] * (r[j] * dinv[j]);
+    }
+    __syncthreads();
+    pair_pass();
+    double v[3] = {0, 0, 0};  // (r,u), (w,u), (r,r)
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int c = cut_of(j);
+      wl[j] = 0.0;
       if (c < m) {
-        x[j] += alpha * p[j];
-        const double rc = r[j] - alpha * q[j];
-        r[j] = rc;
-        v2[0] += rc * rc * dinv[j];
-        v2[1] += rc * rc;
+        const double uc = r[j] * dinv[j];
+        wl[j] = uc + 0.5 * sc[j] * cut_acc(j);
+        v[0] += r[j] * uc;
+        v[1] += wl[j] * uc;
+        v[2] += r[j] * r[j];
       }
     }
-    block_sums<2>(v2, red);
-    const double beta = v2[0] / rz;
-    rz = v2[0];
-    rr = v2[1];
-#pragma unroll
-    for (int j = 0; j < CPT; ++j) p[j] = r[j] * dinv[j] + beta * p[j];
+    reduce(v);
     ++it;
+    rr = v[2];
+    beta = v[0] / gamma;
+    alpha = v[0] / (v[1] - beta * v[0] / alpha);
+    gamma = v[0];
+    run = it < maxit && rr > thr;
   }
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
-    const int c = tid + j * WG;
+    const int c = cut_of(j);
     if (c < m) nd.t[c] = x[j];
   }
   if (tid == 0) {
@@ -880,28 +955,38 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   k_admm_prep<<<grid, WGS, 0, st>>>(nodes);
 }
 
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int maxit,
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, int maxit,
                size_t lds_limit, hipStream_t st) {
   const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
-  if (lds <= lds_limit) {
-    const bool i16 = mmax < 32768 && namax < 32768;
-    if (mmax <= 2 * WG) {
-      if (i16)
-        k_cg_reg<2, short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
-      else
-        k_cg_reg<2, int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
-    } else if (mmax <= 4 * WG) {
-      if (i16)
-        k_cg_reg<4, short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
-      else
-        k_cg_reg<4, int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
-    } else if (i16)
+  const bool i16 = mmax < 32768 && namax < 32768;
+  // index streams staged in LDS as well: aptr (ints) and aent, cact
+  const size_t lds_st = lds + ((size_t)namax + 1) * sizeof(int) +
+                        2 * (size_t)nemax * (i16 ? sizeof(short) : sizeof(int));
+  const bool st_ok = lds_st <= lds_limit;
+#define MADAM_CG1(CPT)                                                              \
+  do {                                                                              \
+    if (i16 && st_ok)                                                               \
+      k_cg1<CPT, short, true><<<nb, WG, lds_st, st>>>(nodes, rtol, maxit);          \
+    else if (i16)                                                                   \
+      k_cg1<CPT, short, false><<<nb, WG, lds, st>>>(nodes, rtol, maxit);            \
+    else if (st_ok)                                                                 \
+      k_cg1<CPT, int, true><<<nb, WG, lds_st, st>>>(nodes, rtol, maxit);            \
+    else                                                                            \
+      k_cg1<CPT, int, false><<<nb, WG, lds, st>>>(nodes, rtol, maxit);              \
+  } while (0)
+  if (lds <= lds_limit && mmax <= 2 * WG) {
+    MADAM_CG1(2);
+  } else if (lds <= lds_limit && mmax <= 4 * WG) {
+    MADAM_CG1(4);
+  } else if (lds <= lds_limit) {
+    if (i16)
       k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
     else
       k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
   } else {
     k_cg<<<nb, WG, 0, st>>>(nodes, rtol, maxit);
   }
+#undef MADAM_CG1
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

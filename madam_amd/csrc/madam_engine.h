@@ -369,6 +369,7 @@ inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
   v.n = n;
   v.m = cs.m;
   v.nact = cs.nact;
+  v.ne = cs.cptr.empty() ? 0 : cs.cptr[cs.m];
   v.L = sl.L.p;
   v.X = sl.X.p;
   v.Q = sl.Q.p;

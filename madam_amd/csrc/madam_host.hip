@@ -40,10 +40,11 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
   if ((rc = E.views.ensure(k))) return rc;
   if ((rc = up(E.views.p, hv.data(), k, st))) return rc;
   const int chunk = std::max(1, o.check_every);
-  int itmax_all = 0, namax = 1;
+  int itmax_all = 0, namax = 1, nemax = 1;
   for (int b = 0; b < k; ++b) {
     itmax_all = std::max(itmax_all, hv[b].itermax);
     namax = std::max(namax, hv[b].nact);
+    nemax = std::max(nemax, hv[b].ne);
   }
   long launched = 0;
   int active = k;
@@ -54,15 +55,18 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     }
     launched += chunk;
     // tile-parallel sign iteration while the active nodes cannot fill the
-    // CUs with one workgroup each
-    // (nodes of order <= 112 keep their iterates in LDS: one workgroup each)
+    // CUs with one workgroup each.  Nodes of order <= 112 keep their
+    // iterates in LDS in the one-workgroup layout, which wins from ~32
+    // active nodes on; below that, and for order > 64, the tile launches
+    // are faster (measured: n = 100, one node, 0.14 ms against 0.25 ms)
     const bool small = madam::ns_lds_bytes(nmax) + 1024 <= E.lds_limit;
     const bool tiles =
-        o.ns_layout == 2 || (o.ns_layout == 0 && !small && active < E.cu_count / 2);
+        o.ns_layout == 2 ||
+        (o.ns_layout == 0 && (small ? (nmax > 64 && active <= 16) : active < E.cu_count / 2));
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
-            madam::launch_cg(E.views.p, k, mmax, namax, o.cg_rtol, o.cg_maxit, E.lds_limit, st));
+            madam::launch_cg(E.views.p, k, mmax, namax, nemax, o.cg_rtol, o.cg_maxit, E.lds_limit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
         TIMED(PH_EIG, st, {

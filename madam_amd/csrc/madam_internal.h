@@ -63,13 +63,13 @@ struct NodeView {
   int ipm_status; // 0 ok, else the reference's failure point (1..4)
   int ns_fail;    // the latest sign-iteration projection failed its guard
   int ns_fallbacks;  // projections redone by FP64 Jacobi after a failed guard
-  int pad2;
+  int ne;         // cptr[m]: pair entries of all cuts (= aptr[nact])
 };
 
 // Launch helpers implemented in madam_device.hip.  All take the device array
 // of nodes and the batch size, and run on `stream`.
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, double rtol, int maxit,
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, int maxit,
                size_t lds_limit, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,

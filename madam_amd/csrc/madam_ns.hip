@@ -64,20 +64,48 @@ __device__ inline void tile_decode(int t, int& i0, int& j0) {
 // kq = n_p / 4 (a multiple of 4).  Lane layout of v_mfma_f32_16x16x4f32:
 // A operand lane l -> (row l%16, k l/16); B operand lane l -> (k l/16,
 // col l%16); result lane l -> rows 4(l/16)+v, col l%16.
+// NS consecutive k-steps of one quarter: all 2 NS operand loads are issued
+// before the first MFMA (one memory latency per NS steps instead of one per
+// step; the accumulation order is unchanged).
+template <int NS>
+__device__ __forceinline__ void ksteps_f32(const float* __restrict__ A, const float* __restrict__ B,
+                                           int np, float alpha, float beta, int ia, int jb,
+                                           int k, f32x4& acc) {
+  float a[NS], b[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    const int kk = k + 4 * t;
+    a[t] = A[ia + (size_t)kk * np];
+    // B symmetric: B[k][j] = B[j + k np]
+    float bb = beta * B[jb + (size_t)kk * np];
+    if (jb == kk) bb += alpha;
+    b[t] = bb;
+  }
+#pragma unroll
+  for (int t = 0; t < NS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[t], acc, 0, 0, 0);
+}
+
 __device__ __forceinline__ f32x4 quarter_f32(const float* __restrict__ A,
                                              const float* __restrict__ B, int np, float alpha,
                                              float beta, int i0, int j0, int qi) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const int kq = np >> 2;
+  const int ia = i0 + r, jb = j0 + r;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = qi * kq; k0 < (qi + 1) * kq; k0 += 4) {
-    const int k = k0 + q;
-    const float a = A[(i0 + r) + (size_t)k * np];
-    // B symmetric: B[k][j] = B[j + k np]
-    float b = beta * B[(j0 + r) + (size_t)k * np];
-    if (j0 + r == k) b += alpha;
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  int k = qi * kq + q, rem = kq >> 2;
+  for (; rem >= 16; rem -= 16, k += 64) ksteps_f32<16>(A, B, np, alpha, beta, ia, jb, k, acc);
+  switch (rem) {
+#define MADAM_KS(N) \
+  case N:           \
+    ksteps_f32<N>(A, B, np, alpha, beta, ia, jb, k, acc); \
+    break;
+    MADAM_KS(1) MADAM_KS(2) MADAM_KS(3) MADAM_KS(4) MADAM_KS(5) MADAM_KS(6) MADAM_KS(7)
+    MADAM_KS(8) MADAM_KS(9) MADAM_KS(10) MADAM_KS(11) MADAM_KS(12) MADAM_KS(13) MADAM_KS(14)
+    MADAM_KS(15)
+#undef MADAM_KS
+    default:
+      break;
   }
   return acc;
 }
@@ -138,6 +166,23 @@ __device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const 
 // (k and tiles past n are masked), quarters of kq = 4 ceil(n / 16).  The
 // result register v of lane l holds row (l/16) + 4v, column l%16
 // (measured on gfx950).
+template <int NS>
+__device__ __forceinline__ void ksteps_f64(const double* __restrict__ A,
+                                           const double* __restrict__ B, int n, double alpha,
+                                           double beta, int ia, int jb, int k, f64x4& acc) {
+  double a[NS], b[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    const int kk = k + 4 * t;
+    a[t] = (ia < n && kk < n) ? A[ia + (size_t)kk * n] : 0.0;
+    double bb = (jb < n && kk < n) ? beta * B[jb + (size_t)kk * n] : 0.0;
+    if (jb == kk && kk < n) bb += alpha;
+    b[t] = bb;
+  }
+#pragma unroll
+  for (int t = 0; t < NS; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], acc, 0, 0, 0);
+}
+
 __device__ __forceinline__ f64x4 quarter_f64(const double* __restrict__ A,
                                              const double* __restrict__ B, int n, double alpha,
                                              double beta, int i0, int j0, int qi) {
@@ -146,12 +191,17 @@ __device__ __forceinline__ f64x4 quarter_f64(const double* __restrict__ A,
   const int kq = ((n + 15) >> 4) << 2;
   const int ia = i0 + r, jb = j0 + r;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = qi * kq; k0 < (qi + 1) * kq; k0 += 4) {
-    const int k = k0 + q;
-    const double a = (ia < n && k < n) ? A[ia + (size_t)k * n] : 0.0;
-    double b = (jb < n && k < n) ? beta * B[jb + (size_t)k * n] : 0.0;
-    if (jb == k && k < n) b += alpha;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  int k = qi * kq + q, rem = kq >> 2;
+  for (; rem >= 8; rem -= 8, k += 32) ksteps_f64<8>(A, B, n, alpha, beta, ia, jb, k, acc);
+  switch (rem) {
+#define MADAM_KS(N) \
+  case N:           \
+    ksteps_f64<N>(A, B, n, alpha, beta, ia, jb, k, acc); \
+    break;
+    MADAM_KS(1) MADAM_KS(2) MADAM_KS(3) MADAM_KS(4) MADAM_KS(5) MADAM_KS(6) MADAM_KS(7)
+#undef MADAM_KS
+    default:
+      break;
   }
   return acc;
 }
