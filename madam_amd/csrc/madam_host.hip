@@ -55,14 +55,19 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     }
     launched += chunk;
     // tile-parallel sign iteration while the active nodes cannot fill the
-    // CUs with one workgroup each.  Nodes of order <= 112 keep their
-    // iterates in LDS in the one-workgroup layout, which wins from ~32
-    // active nodes on; below that, and for order > 64, the tile launches
-    // are faster (measured: n = 100, one node, 0.14 ms against 0.25 ms)
+    // CUs with one workgroup each.  Crossovers measured per projection:
+    // order <= 112 (iterates in LDS in the one-workgroup layout, flat
+    // 0.25 ms at n_p = 112, 0.125 ms at 80, 0.093 ms at 64): tiles up to
+    // 96 active nodes at n_p = 112 (0.13 ms for one node), 16 at n_p = 80,
+    // never at 64; larger orders: tiles below ~3/4 of the CU count (n = 150:
+    // 1.0 against 1.13 ms at 192 nodes, 1.31 against 1.20 ms at 256)
     const bool small = madam::ns_lds_bytes(nmax) + 1024 <= E.lds_limit;
+    const int np = (nmax + 15) & ~15;
+    const int small_tiles = np >= 112 ? 96 : (np >= 96 ? 48 : (np >= 80 ? 16 : 0));
     const bool tiles =
         o.ns_layout == 2 ||
-        (o.ns_layout == 0 && (small ? (nmax > 64 && active <= 16) : active < E.cu_count / 2));
+        (o.ns_layout == 0 &&
+         (small ? active <= small_tiles : active < (3 * E.cu_count) / 4));
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,

@@ -371,13 +371,24 @@ __global__ __launch_bounds__(1024) void k_ns_init(NodeView* nodes) {
 // four waves per tile, wave w accumulating K quarter w (split K: a quarter
 // of the latency of one wave doing the whole tile), the partial tiles summed
 // in LDS in the quarter order of gemm_tile_f32.
-__global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int ia, int ib, int ic,
+// Grids of the per-tile kernels are (8 x tiles, ceil(nodes / 8)): workgroup
+// (x, y) serves node 8 y + x % 8 and tile x / 8.  Workgroups go to the 8
+// XCDs round-robin in dispatch order, so every tile of a node runs on the
+// same XCD and the node's operand panels are fetched into one L2 only.
+__device__ __forceinline__ int xcd_node(int nb) {
+  const int b = blockIdx.y * 8 + (blockIdx.x & 7);
+  return b < nb ? b : -1;
+}
+
+__global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int ia, int ib, int ic,
                                                    float alpha, float beta) {
   __shared__ f32x4 part[3][64];
-  NodeView& nd = nodes[blockIdx.y];
+  const int b = xcd_node(nb);
+  if (b < 0) return;
+  NodeView& nd = nodes[b];
   if (nd.done) return;
   const int np = (nd.n + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x;
+  const int t = blockIdx.x >> 3;
   if (t >= ntile) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i0, j0;
@@ -393,13 +404,16 @@ __global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int ia, int 
 }
 
 // FP32 iterate (buffer is) -> FP64 n x n in S2, with the guard.
-__global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int is) {
-  NodeView& nd = nodes[blockIdx.y];
+__global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int is) {
+  const int b = xcd_node(nb);
+  if (b < 0) return;
+  NodeView& nd = nodes[b];
   if (nd.done) return;
   const int n = nd.n, np = (n + 15) & ~15;
   const float* S = nd.ns32 + (size_t)is * np * np;
   int bad = 0;
-  for (int f = blockIdx.x * 256 + threadIdx.x; f < n * n; f += gridDim.x * 256) {
+  const int chunks = gridDim.x >> 3;
+  for (int f = (blockIdx.x >> 3) * 256 + threadIdx.x; f < n * n; f += chunks * 256) {
     const int j = f / n, i = f - j * n;
     const float v = S[i + (size_t)j * np];
     if (!(fabsf(v) <= 1.5f)) bad = 1;
@@ -410,12 +424,14 @@ __global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int is) {
 
 // The FP64 finish, one product per launch (split K as above): 0: S1 = S2^2;
 // 1: ip1 = S2 (1.5 I - 0.5 S1); 2: S1 = W (0.5 I + 0.5 ip1) = X_+.
-__global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int which) {
+__global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int which) {
   __shared__ f64x4 part[3][64];
-  NodeView& nd = nodes[blockIdx.y];
+  const int b = xcd_node(nb);
+  if (b < 0) return;
+  NodeView& nd = nodes[b];
   if (nd.done || nd.ns_fail) return;
   const int n = nd.n, nt = (n + 15) >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x;
+  const int t = blockIdx.x >> 3;
   if (t >= ntile) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i0, j0;
@@ -434,22 +450,23 @@ __global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int which) {
 void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   static const float coef[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
   const int np = (nmax + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
-  const dim3 g32(ntile, nb);
-  k_ns_init<<<nb, 1024, 0, st>>>(nodes);
+  const int ny = (nb + 7) / 8;
+  const dim3 g32(8 * ntile, ny);
+  k_ns_init<<<nb, 1024, 0, st>>>(nodes);  // workgroup b on XCD b % 8 as well
   int sb = 0, tb = 1, ub = 2;
   for (int k = 0; k < MADAM_NS_STEPS; ++k) {
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, sb, sb, tb, 0.f, 1.f);               // T = S^2
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, tb, tb, ub, coef[k][1], coef[k][2]);  // U
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, sb, ub, tb, coef[k][0], 1.f);         // S'
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, sb, sb, tb, 0.f, 1.f);               // T = S^2
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, tb, tb, ub, coef[k][1], coef[k][2]);  // U
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, sb, ub, tb, coef[k][0], 1.f);         // S'
     const int x = sb;
     sb = tb;
     tb = x;
   }
   const int nconv = std::min(64, (nmax * nmax + 255) / 256);
-  k_ns_conv<<<dim3(nconv, nb), 256, 0, st>>>(nodes, sb);
+  k_ns_conv<<<dim3(8 * nconv, ny), 256, 0, st>>>(nodes, nb, sb);
   const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
-  const dim3 g64(ntile64, nb);
-  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 256, 0, st>>>(nodes, w);
+  const dim3 g64(8 * ntile64, ny);
+  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 256, 0, st>>>(nodes, nb, w);
 }
 
 // ---------------------------------------------------------------------------
