@@ -5,40 +5,114 @@
 // Per node: X (nd.X), Z = Diag(y) - L (nd.Q), y (nd.y); scratch Zi (S1),
 // M (S2), dX (W), dX1 (ip1), T1 (ip2), T2 (ip3), vectors in ipv.  Cholesky
 // factors are packed lower triangles, in LDS when they fit, else in ipP.
-// Where the reference exits (ipm_mc_pk.c:124-135, 161-165, 218-222) the
-// kernel stops with ipm_status = 1..4 and the host reports the failure.
+// Where the reference exits (ipm_mc_pk.c:124-135, 161-165) the kernel
+// stops with ipm_status = 1..3 and the host reports the failure.  Zi is
+// formed as the reference does (dpotrf + dpotri, ipm_mc_pk.c:121-135); the
+// two dposv solves with the same M = Zi .* X (ipm_mc_pk.c:154-158,
+// 208-217) share one factorisation, which is the one dposv would compute
+// again.
 #include "madam_devutil.h"
 #include "madam_internal.h"
 
 namespace madam {
 
-// Factor the packed lower triangle P (order n, leading order n2) in place:
-// P = L with L L^T = A.  Returns 0 or j+1 at the first non-positive pivot.
+// Right-looking Cholesky of the packed lower triangle P (order n, leading
+// order n2) in place, one barrier per column: in step j wave w subtracts
+// column j's outer product from trailing columns k = j+1+w, j+1+w+nw, ...
+// (lanes down the column: contiguous LDS); the wave that owns column j+1
+// then finishes it (square root of its diagonal, column scaled), so it is
+// the pivot column of step j+1.  Returns 0, or c+1 for the first column c
+// with a non-positive pivot -- the same value in every thread.  flag: two
+// shared ints; a failure in step j is posted in flag[(j+1) & 1], which is
+// not written again before every thread has read it.
 __device__ int chol_packed(double* P, int n, int n2, int* flag) {
-  const int tid = threadIdx.x;
-  for (int j = 0; j < n; ++j) {
-    const double djj = P[pk(j, j, n2)];
-    if (!(djj > 0.0)) return j + 1;  // uniform: every thread read the same value
-    const double d = sqrt(djj);
-    __syncthreads();
-    if (tid == 0) P[pk(j, j, n2)] = d;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  __syncthreads();  // readers of flag in a previous call are done
+  if (wv == 0) {
+    if (lane == 0) flag[0] = flag[1] = 0;
+    const double d = P[0];
+    if (d > 0.0) {
+      const double sd = sqrt(d);
+      for (int i = 1 + lane; i < n; i += 64) P[i] /= sd;  // pk(i, 0) = i
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) P[0] = sd;
+    } else if (lane == 0) {
+      flag[0] = 1;
+    }
+  }
+  __syncthreads();
+  if (flag[0]) return 1;
+  for (int j = 0; j + 1 < n; ++j) {
     const int base = pk(j, j, n2);
-    for (int i = j + 1 + tid; i < n; i += blockDim.x) P[base + (i - j)] /= d;
-    __syncthreads();
-    const int m = n - j - 1;
-    const int npair = m * (m + 1) / 2;
-    for (int f = tid; f < npair; f += blockDim.x) {
-      int a = (int)((sqrtf(8.0f * (float)f + 1.0f) - 1.0f) * 0.5f);
-      if ((a + 1) * (a + 2) / 2 <= f) ++a;
-      if (a * (a + 1) / 2 > f) --a;
-      const int b = f - a * (a + 1) / 2;
-      const int i = j + 1 + a, k = j + 1 + b;
-      P[pk(i, k, n2)] -= P[base + (i - j)] * P[base + (k - j)];
+    for (int k = j + 1 + wv; k < n; k += nw) {
+      const double lk = P[base + (k - j)];
+      const int ck = pk(k, k, n2);
+      for (int i = k + lane; i < n; i += 64) P[ck + (i - k)] -= P[base + (i - j)] * lk;
+      if (k == j + 1) {
+        // the wave's own LDS writes are seen by its later reads (in order)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const double d = P[ck];
+        if (d > 0.0) {
+          const double sd = sqrt(d);
+          for (int i = k + 1 + lane; i < n; i += 64) P[ck + (i - k)] /= sd;
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (lane == 0) P[ck] = sd;
+        } else if (lane == 0) {
+          flag[(j + 1) & 1] = k + 1;
+        }
+      }
     }
     __syncthreads();
+    const int f = flag[(j + 1) & 1];
+    if (f) return f;
   }
-  (void)flag;
   return 0;
+}
+
+// A^{-1} = L^{-T} L^{-1} from the packed Cholesky factor L of A (dpotri):
+// L^{-1} in place in P, column by column from the right (dtrti2 order:
+// column j below the diagonal becomes -(1/L_jj) T x with T the inverted
+// trailing block, one output row per group of 8 lanes), then out = the
+// full symmetric A^{-1} (ld n), each lower entry an independent dot
+// product of two columns of L^{-1}.  tmp: n - 1 doubles (the caller's
+// 16 x 65 LDS slab: n <= 1041).
+__device__ void potri_packed(double* P, int n, int n2, double* out, double* tmp) {
+  const int tid = threadIdx.x, g = tid & 7, grp = tid >> 3, ngrp = blockDim.x >> 3;
+  for (int j = n - 1; j >= 0; --j) {
+    const int base = pk(j, j, n2);
+    const double ajj = 1.0 / P[base];
+    const int m = n - 1 - j;
+    for (int i0 = 0; i0 < m; i0 += ngrp) {
+      const int ii = i0 + grp;  // the same in all 8 lanes of a group
+      double acc = 0.0;
+      if (ii < m) {
+        const int i = j + 1 + ii;
+        for (int l = j + 1 + g; l <= i; l += 8) acc += P[pk(i, l, n2)] * P[base + (l - j)];
+      }
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      acc += __shfl_xor(acc, 4, 64);
+      if (ii < m && g == 0) tmp[ii] = acc;
+    }
+    __syncthreads();
+    for (int ii = tid; ii < m; ii += blockDim.x) P[base + 1 + ii] = -ajj * tmp[ii];
+    if (tid == 0) P[base] = ajj;
+    __syncthreads();
+  }
+  const int npair = n * (n + 1) / 2;
+  for (int f = tid; f < npair; f += blockDim.x) {
+    int i, j;
+    tri_decode(f, i, j);  // i >= j
+    const int ci = pk(i, i, n2), cj = pk(j, j, n2);
+    double acc = 0.0;
+    for (int k = i; k < n; ++k) acc += P[ci + (k - i)] * P[cj + (k - j)];
+    out[i + (size_t)j * n] = acc;
+    out[j + (size_t)i * n] = acc;
+  }
+  __syncthreads();
 }
 
 // P[lower] = A + alpha * Diag(dv) (dv may be null) or A + alpha * B.
@@ -55,9 +129,74 @@ __device__ void load_packed(double* P, int n, int n2, const double* A, double al
   __syncthreads();
 }
 
-// Solve (L L^T) x = b with the packed factor; wave 0 does the work, b and x
-// may alias.  Row-oriented sweeps with wave reductions.
+// Solve (L L^T) x = b with the packed factor, by wave 0 (b and x may
+// alias).  Column-oriented sweeps in LAPACK dtrsv's order (forward: x_k =
+// b_k / L_kk, then b_i -= L_ik x_k below; backward: x_k = (z_k - sum over
+// i > k of L_ik x_i, i descending) / L_kk): row i lives in a register of
+// lane i % 64 and each x_k is broadcast from its lane, so a step costs a
+// few LDS reads and no reduction.  n <= 256.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ void chol_solve_reg(const double* P, int n, int n2, const double* b, double* x) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    double v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = lane + 64 * r;
+      v[r] = i < n ? b[i] : 0.0;
+    }
+    for (int k = 0; k < n; ++k) {
+      const int src = k & 63, rk = k >> 6;
+      double vk = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r == rk) vk = v[r];
+      const double xk = lane_bcast(vk, src) / P[pk(k, k, n2)];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = lane + 64 * r;
+        if (i == k)
+          v[r] = xk;
+        else if (i > k && i < n)
+          v[r] -= P[pk(i, k, n2)] * xk;
+      }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      const int src = k & 63, rk = k >> 6;
+      double vk = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r == rk) vk = v[r];
+      const double xk = lane_bcast(vk, src) / P[pk(k, k, n2)];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = lane + 64 * r;
+        if (i == k)
+          v[r] = xk;
+        else if (i < k)
+          v[r] -= P[pk(k, i, n2)] * xk;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = lane + 64 * r;
+      if (i < n) x[i] = v[r];
+    }
+  }
+  __syncthreads();
+}
+
+// The same for any n: row-oriented sweeps with wave reductions.
 __device__ void chol_solve1(const double* P, int n, int n2, const double* b, double* x) {
+  if (n <= 256) {
+    chol_solve_reg(P, n, n2, b, x);
+    return;
+  }
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
     for (int i = 0; i < n; ++i) {
@@ -97,7 +236,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   double* red = smem;
   double* tA = red + 64;
   double* tB = tA + 16 * 65;
-  __shared__ int flag;
+  __shared__ int flag[2];
   double* P = P_LDS ? (tB + 16 * 65) : nd.ipP;
   const double* __restrict__ L = nd.L;
   double* X = nd.X;
@@ -111,7 +250,6 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   double* T2 = nd.ip3;
   double* dy = nd.ipv;
   double* dy1 = nd.ipv + n;
-  double* dd = nd.ipv + 2 * n;
   const size_t nn = (size_t)n * n;
 
   // y = 1.1 + column sums of |L| (ipm_mc_pk.c:61-68); X = I; Z = Diag(y) - L
@@ -138,36 +276,18 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   int it = 0, status = 0;
   while (phi - psi > 1e-4 && it < maxit) {
     ++it;
-    // ---- Zi = Z^{-1}: factor, then solve L L^T Zi = I column by column
+    // ---- Zi = Z^{-1}: factor, then invert from the factor
     load_packed(P, n, n2, Z, 0.0, nullptr, nullptr);
-    if (chol_packed(P, n, n2, &flag)) { status = 1; break; }
-    for (int c = tid; c < n; c += WG) {
-      double* col = Zi + (size_t)c * n;
-      for (int i = 0; i < c; ++i) col[i] = 0.0;
-      for (int i = c; i < n; ++i) {
-        double acc = (i == c) ? 1.0 : 0.0;
-        for (int k = c; k < i; ++k) acc -= P[pk(i, k, n2)] * col[k];
-        col[i] = acc / P[pk(i, i, n2)];
-      }
-      for (int i = n - 1; i >= 0; --i) {
-        double acc = col[i];
-        for (int k = i + 1; k < n; ++k) acc -= P[pk(k, i, n2)] * col[k];
-        col[i] = acc / P[pk(i, i, n2)];
-      }
-    }
-    __syncthreads();
-    for (size_t f = tid; f < nn; f += WG) {  // symmetric from the lower triangle
-      const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
-      if (i < j) Zi[f] = Zi[j + (size_t)i * n];
-    }
-    __syncthreads();
+    if (chol_packed(P, n, n2, flag)) { status = 1; break; }
+    potri_packed(P, n, n2, Zi, tA);
     // ---- predictor: dy1 = (Zi .* X) \ (-e)
     for (size_t f = tid; f < nn; f += WG) Mm[f] = Zi[f] * X[f];
-    for (int i = tid; i < n; i += WG) dy1[i] = -1.0;
     __syncthreads();
     load_packed(P, n, n2, Mm, 0.0, nullptr, nullptr);
-    if (chol_packed(P, n, n2, &flag)) { status = 3; break; }
-    chol_solve1(P, n, n2, dy1, dy1);
+    if (chol_packed(P, n, n2, flag)) { status = 3; break; }
+    for (int i = tid; i < n; i += WG) dy1[i] = -1.0;
+    __syncthreads();
+    chol_solve1(P, n, n2, dy1, dy1);  // P keeps M's factor for the corrector
     // dX1 = -Zi Diag(dy1) X - X, symmetrised
     for (size_t f = tid; f < nn; f += WG) {
       const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
@@ -199,9 +319,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       dy[i] = -acc + mu * Zi[(size_t)i * (n + 1)];
     }
     __syncthreads();
-    load_packed(P, n, n2, Mm, 0.0, nullptr, nullptr);
-    if (chol_packed(P, n, n2, &flag)) { status = 4; break; }
-    chol_solve1(P, n, n2, dy, dy);
+    chol_solve1(P, n, n2, dy, dy);  // the same M: its factor is still in P
     // dX2 = mu Zi - Zi (Diag(dy2) X + Diag(dy1) dX1)
     for (size_t f = tid; f < nn; f += WG) {
       const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
@@ -229,14 +347,14 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
     double ap = 1.0;
     for (int tries = 0; tries < 400; ++tries) {
       load_packed(P, n, n2, X, ap, dX, nullptr);
-      if (!chol_packed(P, n, n2, &flag)) break;
+      if (!chol_packed(P, n, n2, flag)) break;
       ap *= 0.8;
     }
     if (ap < 1.0) ap *= 0.95;
     double ad = 1.0;
     for (int tries = 0; tries < 400; ++tries) {
       load_packed(P, n, n2, Z, ad, nullptr, dy);
-      if (!chol_packed(P, n, n2, &flag)) break;
+      if (!chol_packed(P, n, n2, flag)) break;
       ad *= 0.8;
     }
     if (ad < 1.0) ad *= 0.95;
@@ -260,7 +378,6 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
     phi = w3[2];
     psi = w3[1];
   }
-  (void)dd;
   if (tid == 0) {
     nd.f = phi;
     nd.ipm_iters = it;

@@ -237,7 +237,22 @@ __device__ __forceinline__ int rand31(uint64_t base, uint64_t& ctr) {
 // nodes), so that a workgroup's runs share one X.  With XL the workgroup
 // first copies X into LDS (n^2 doubles after the permutation buffers) and
 // every run reads it there: same values, same trajectory, LDS latency.
-template <bool XL>
+// x mod n for 0 <= x < 2^31 by a double-precision reciprocal and one
+// correction (the quotient estimate is off by at most one): exact, and
+// cheaper than the integer division of a run-time divisor.
+__device__ __forceinline__ int mod_n(int x, int n, double inv_n) {
+  const int q = (int)((double)x * inv_n);
+  int r = x - q * n;
+  if (r < 0)
+    r += n;
+  else if (r >= n)
+    r -= n;
+  return r;
+}
+
+// K: the cut order of every run of the launch (5 or 7; 0 = read from the
+// job), so that the loops over the permutation's first K entries unroll.
+template <bool XL, int K>
 __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, SaOut* out) {
 #pragma clang fp contract(off)
   extern __shared__ unsigned short permbuf[];
@@ -255,7 +270,8 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
   }
   if (rr0 >= per) return;
   const SaJob J = jobs[r];
-  const int n = J.n, k = J.k;
+  const int n = J.n, k = K ? K : J.k;
+  const double inv_n = 1.0 / (double)n;
   unsigned short* perm = permbuf + (size_t)threadIdx.x * nst;
   uint64_t base = splitmix64(J.seed ^ splitmix64((uint64_t)J.stream_id));
   uint64_t ctr = 0;
@@ -286,7 +302,7 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
     changed = 0;
     for (int it = 0; it < m1; ++it) {
       int i1 = rand31(base, ctr) % k;
-      int i2 = rand31(base, ctr) % n;
+      int i2 = mod_n(rand31(base, ctr), n, inv_n);
       if (i2 < i1) {
         const int sw = i1;
         i1 = i2;
@@ -294,15 +310,18 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
       }
       const int p1 = perm[i1], p2 = perm[i2];
       double delta = 0.0;
-      for (int i = 0; i < k; ++i) {
+#pragma unroll
+      for (int i = 0; i < (K ? K : k); ++i) {
         const int pi = perm[i];
         delta += (SG(i) * SG(i1)) * (XV(pi, p2) - XV(pi, p1));
       }
-      if (i2 < k)
-        for (int i = 0; i < k; ++i) {
+      if (i2 < k) {
+#pragma unroll
+        for (int i = 0; i < (K ? K : k); ++i) {
           const int pi = perm[i];
           delta += (SG(i) * SG(i2)) * (XV(pi, p1) - XV(pi, p2));
         }
+      }
       delta *= 2.0;
       const double d1 = XV(p1, p1) + XV(p2, p2) - 2.0 * XV(p2, p1);
       double d2 = 1.0;
@@ -361,15 +380,30 @@ void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipS
   k_sumx<<<(nb + 63) / 64, 64, 0, st>>>(X, n, out, nb);
 }
 
-void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, SaOut* out, size_t lds_limit,
-                   hipStream_t st) {
+void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, int kk, SaOut* out,
+                   size_t lds_limit, hipStream_t st) {
   const dim3 grid((per + SA_WG - 1) / SA_WG, nnodes);
   const size_t lperm = (((size_t)SA_WG * sa_nstride(nmax) + 3) & ~(size_t)3) * sizeof(unsigned short);
   const size_t lx = (size_t)nmax * nmax * sizeof(double);
-  if (lperm + lx <= lds_limit)
-    k_qap_sa<true><<<grid, SA_WG, lperm + lx, st>>>(jobs, per, out);
-  else
-    k_qap_sa<false><<<grid, SA_WG, lperm, st>>>(jobs, per, out);
+  const bool xl = lperm + lx <= lds_limit;
+  const size_t lds = xl ? lperm + lx : lperm;
+#define MADAM_SA(XLV, KV) k_qap_sa<XLV, KV><<<grid, SA_WG, lds, st>>>(jobs, per, out)
+  if (xl) {
+    if (kk == 5)
+      MADAM_SA(true, 5);
+    else if (kk == 7)
+      MADAM_SA(true, 7);
+    else
+      MADAM_SA(true, 0);
+  } else {
+    if (kk == 5)
+      MADAM_SA(false, 5);
+    else if (kk == 7)
+      MADAM_SA(false, 7);
+    else
+      MADAM_SA(false, 0);
+  }
+#undef MADAM_SA
 }
 
 }  // namespace madam
