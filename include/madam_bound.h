@@ -70,6 +70,9 @@ typedef struct {
                               iteration (FP32 MFMA quintic Newton-Schulz + FP64 finish) */
     int ns_layout;         /* sign iteration: 0 automatic, 1 one workgroup per node,
                               2 tile-parallel (each product spread over many workgroups) */
+    double cg_adapt;       /* > 0: after an ADMM call's first iteration, PCG stops at the
+                              relative residual max(cg_rtol, min(1e-3, cg_adapt * the
+                              previous iteration's ADMM error)); 0: cg_rtol throughout */
 } madam_opts;
 
 /* One ADMM_solver call (admm.c:49): inputs are read, outputs written.
@@ -231,6 +234,7 @@ typedef struct {
                                      * against its own heuristic's best only, as separate
                                      * SDPbound calls would; LB ends as the best of them.
                                      * madam_bab ignores it (one shared incumbent). */
+    double cg_adapt;                /* as madam_opts.cg_adapt */
 } madam_bab_ctx;
 
 /* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n

@@ -49,7 +49,7 @@ class Hept(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("cg_rtol", C.c_double), ("cg_maxit", C.c_int),
                 ("jacobi_sweeps", C.c_int), ("check_every", C.c_int), ("projection", C.c_int),
-                ("ns_layout", C.c_int)]
+                ("ns_layout", C.c_int), ("cg_adapt", C.c_double)]
 
 
 class AdmmJob(C.Structure):
@@ -234,13 +234,15 @@ def _make_job(L, X, y, Q, tri, pent, hept, sigma, tol, itermax, scaleSigma, keep
 
 
 def _opts(cg_rtol=1e-12, cg_maxit=1000, jacobi_sweeps=40, check_every=16, projection=0,
-          ns_layout=0):
+          ns_layout=0, cg_adapt=0.0):
     """projection: 0 FP64 eigen (Jacobi), 1 mixed-precision sign iteration;
-    ns_layout: 0 automatic, 1 one workgroup per node, 2 tile-parallel."""
+    ns_layout: 0 automatic, 1 one workgroup per node, 2 tile-parallel;
+    cg_adapt: > 0 loosens the PCG tolerance to cg_adapt x the ADMM error."""
     o = Opts()
     o.cg_rtol, o.cg_maxit, o.jacobi_sweeps, o.check_every = cg_rtol, cg_maxit, jacobi_sweeps, check_every
     o.projection = projection
     o.ns_layout = ns_layout
+    o.cg_adapt = cg_adapt
     return o
 
 
@@ -373,7 +375,7 @@ class BabCtx(C.Structure):
         ("seconds_ipm", C.c_double), ("seconds_admm", C.c_double), ("seconds_sep", C.c_double),
         ("seconds_heur", C.c_double), ("cg_rtol", C.c_double), ("cg_iters", C.c_longlong),
         ("ns_fallbacks", C.c_longlong), ("verbose", C.c_int), ("ns_layout", C.c_int),
-        ("own_incumbent", C.c_int),
+        ("own_incumbent", C.c_int), ("cg_adapt", C.c_double),
     ]
 
 
@@ -444,7 +446,7 @@ class Bounder:
     """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
 
     def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0, cg_rtol=0.0,
-                 verbose=0, own_incumbent=0, ns_layout=0):
+                 verbose=0, own_incumbent=0, ns_layout=0, cg_adapt=0.0):
         self.SPL = _f64(SPL)
         self.ctx = BabCtx()
         self.ctx.N = self.SPL.shape[0]
@@ -460,6 +462,7 @@ class Bounder:
         self.ctx.verbose = verbose
         self.ctx.own_incumbent = own_incumbent
         self.ctx.ns_layout = ns_layout
+        self.ctx.cg_adapt = cg_adapt
 
     def sdpbound(self, nodes, ranks=None):
         k = len(nodes)

@@ -426,6 +426,7 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   o.check_every = 16;
   o.projection = c->projection;
   o.ns_layout = c->ns_layout;
+  o.cg_adapt = c->cg_adapt;
   if ((rc = admm_loop(hv, o, nmax, mmax))) return rc;
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];

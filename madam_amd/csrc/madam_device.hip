@@ -73,11 +73,20 @@ __device__ void spmv_bbt(const NodeView& nd, const double* __restrict__ p, doubl
   __syncthreads();
 }
 
+// Stopping tolerance of a cut solve: rtol, or with adapt > 0 after the
+// first iteration of an ADMM call max(rtol, min(1e-3, adapt * the previous
+// iteration's ADMM error)) -- an inexact solve while ADMM is far from
+// converged, the full tolerance as it gets there.
+__device__ __forceinline__ double cg_tol(const NodeView& nd, double rtol, double adapt) {
+  if (adapt > 0.0 && nd.nbit > 0) return fmax(rtol, fmin(1e-3, adapt * nd.err));
+  return rtol;
+}
+
 // Solve (BB^T + I) t = rhs by conjugate gradients with the diagonal as
 // preconditioner, starting from the current t (the previous iterate), to a
 // relative residual of rtol.  Replaces the CHOLMOD factor/solve of
 // admm.c:486-551.
-__global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int maxit) {
+__global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double adapt, int maxit) {
   __shared__ double red[4 * 16];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
@@ -113,7 +122,8 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, int max
   block_sums<3>(v3, red);
   double rz = v3[0], rr = v3[1];
   const double bb = v3[2];
-  const double thr = rtol * rtol * bb;
+  const double rt = cg_tol(nd, rtol, adapt);
+  const double thr = rt * rt * bb;
   int it = 0;
   while (it < maxit && rr > thr && bb > 0.0) {
     spmv_bbt(nd, p, q, nd.cg_w);
@@ -184,7 +194,7 @@ __device__ inline void bbt_lds(const NodeView& nd, const Idx* __restrict__ aent,
 }
 
 template <class Idx>
-__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int maxit) {
+__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, double adapt, int maxit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
@@ -232,7 +242,8 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, int
   block_sums<3>(v3, red);
   double rz = v3[0], rr = v3[1];
   const double bb = v3[2];
-  const double thr = rtol * rtol * bb;
+  const double rt = cg_tol(nd, rtol, adapt);
+  const double thr = rt * rt * bb;
   int it = 0;
   while (it < maxit && rr > thr && bb > 0.0) {
     bbt_lds(nd, aent, cact, p, q, ps, w);
@@ -309,7 +320,7 @@ __device__ __forceinline__ double signed_gather(const Idx* __restrict__ ent,
 //   w = A u, gamma' = (r,u), delta = (w,u), beta = gamma' / gamma,
 //   alpha = gamma' / (delta - beta gamma' / alpha)
 template <int CPT, class Idx, bool STAGE>
-__global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, int maxit) {
+__global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double adapt, int maxit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (nd.done) return;
@@ -417,7 +428,8 @@ __global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, int ma
   reduce(v3);
   double gamma = v3[0], rr = v3[1];
   const double bb = v3[2];
-  const double thr = rtol * rtol * bb;
+  const double rt = cg_tol(nd, rtol, adapt);
+  const double thr = rt * rt * bb;
   double alpha = 0.0, beta = 0.0;
   int it = 0;
   bool run = it < maxit && rr > thr && bb > 0.0;
@@ -785,16 +797,23 @@ __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only,
   }
 }
 
-// After k_jacobi formed X_+ in S1: X = sigma X_+, Q = X_+ - W and the
-// residuals, sigma update and stop test of admm.c:580-702.
-__global__ __launch_bounds__(WG) void k_admm_post(NodeView* nodes) {
-  __shared__ double red[4 * 16];
-  NodeView& nd = nodes[blockIdx.x];
+// After the projection formed X_+ in S1: X = sigma X_+, Q = X_+ - W and
+// the residuals, sigma update and stop test of admm.c:580-702, in two
+// launches: k_post_part spreads the n^2 and cut
+// passes over a (chunk, node) grid and leaves per-workgroup partial sums
+// (B(X) reads sigma X_+ from S1, the value X receives, so no workgroup
+// waits for another's writes); k_post_fin adds them in workgroup order and
+// updates the node's scalars.
+__global__ __launch_bounds__(WGS) void k_post_part(NodeView* nodes, double* part) {
+  __shared__ double red[8 * 16];
+  NodeView& nd = nodes[blockIdx.y];
   if (nd.done) return;
   const int n = nd.n, m = nd.m, tid = threadIdx.x;
+  const int g = blockIdx.x, G = gridDim.x, stride = G * WGS;
   const double sg = nd.sigma;
-  double v3[3] = {0, 0, 0};  // |X+ - Xold/sigma|_F^2, |diag X - 1|^2, <L,X>
-  for (int f = tid; f < n * n; f += WG) {
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};  // |X+ - Xold/sigma|^2, |diag X - 1|^2, <L,X>,
+                                        // |max(B(X)-1,0)|^2, |s-t|^2, sum t, sum y
+  for (int f = g * WGS + tid; f < n * n; f += stride) {
     const int i = f % n, j = f / n;
     if (i < j) continue;
     const int lo = i + j * n, up = j + i * n;
@@ -808,54 +827,62 @@ __global__ __launch_bounds__(WG) void k_admm_post(NodeView* nodes) {
     nd.Q[lo] = qv;
     nd.Q[up] = qv;
     if (i == j) {
-      v3[0] += dd * dd;
-      v3[1] += (xn - 1.0) * (xn - 1.0);
-      v3[2] += nd.L[lo] * xn;
+      v[0] += dd * dd;
+      v[1] += (xn - 1.0) * (xn - 1.0);
+      v[2] += nd.L[lo] * xn;
     } else {
-      v3[0] += 2.0 * dd * dd;
-      v3[2] += 2.0 * nd.L[lo] * xn;
+      v[0] += 2.0 * dd * dd;
+      v[2] += 2.0 * nd.L[lo] * xn;
     }
   }
-  block_sums<3>(v3, red);
-  // cut residuals: |max(B(X)-1,0)|^2, |s-t|^2, sum t ; and sum y
-  double v4[4] = {0, 0, 0, 0};
-  for (int c = tid; c < m; c += WG) {
+  for (int c = g * WGS + tid; c < m; c += stride) {
     double acc = 0.0;
     for (int e = nd.cptr[c]; e < nd.cptr[c + 1]; ++e) {
       const int en = nd.cent[e];
-      const double v = nd.X[ent_idx(en)];
-      acc = en >= 0 ? acc + v : acc - v;
+      const double x = sg * nd.S1[ent_idx(en)];
+      acc = en >= 0 ? acc + x : acc - x;
     }
     const double bx = -1.0 + nd.cscale[c] * acc;
-    if (bx > 0) v4[0] += bx * bx;
+    if (bx > 0) v[3] += bx * bx;
     const double ds = nd.s[c] - nd.t[c];
-    v4[1] += ds * ds;
-    v4[2] += nd.t[c];
+    v[4] += ds * ds;
+    v[5] += nd.t[c];
   }
-  for (int i = tid; i < n; i += WG) v4[3] += nd.y[i];
-  block_sums<4>(v4, red);
-  if (tid == 0) {
-    nd.sigma_last = sg;
-    const double err_p = (sqrt(v3[1]) + sqrt(v4[0])) / (1.0 + sqrt((double)n));
-    const double err_d = (sqrt(v4[1]) + sqrt(v3[0])) / (1.0 + nd.normL);
-    const double primal = v3[2];
-    const double dual = v4[3] + v4[2];
-    const double gap = fabs(dual - primal) / (1.0 + fabs(dual) + fabs(primal));
-    const double err = err_p > err_d ? (err_p > gap ? err_p : gap) : (err_d > gap ? err_d : gap);
-    const double dgp = log10(err_p), dgd = log10(err_d);
-    if (dgd - dgp < -0.5)
-      nd.sigma = sg / nd.scaleSigma;
-    else if (dgd - dgp > 0.5)
-      nd.sigma = sg * nd.scaleSigma;
-    nd.err_p = err_p;
-    nd.err_d = err_d;
-    nd.gap = gap;
-    nd.err = err;
-    nd.primal = primal;
-    nd.dual = dual;
-    nd.nbit += 1;
-    if (err < nd.tol || nd.nbit > nd.itermax) nd.done = 1;
-  }
+  if (g == 0)
+    for (int i = tid; i < n; i += WGS) v[6] += nd.y[i];
+  block_sums<7>(v, red);
+  if (tid == 0)
+    for (int q = 0; q < 7; ++q) part[((size_t)blockIdx.y * 64 + g) * 8 + q] = v[q];
+}
+
+__global__ __launch_bounds__(64) void k_post_fin(NodeView* nodes, const double* part, int G) {
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done || threadIdx.x != 0) return;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int g = 0; g < G; ++g)
+    for (int q = 0; q < 7; ++q) v[q] += part[((size_t)blockIdx.x * 64 + g) * 8 + q];
+  const int n = nd.n;
+  const double sg = nd.sigma;
+  nd.sigma_last = sg;
+  const double err_p = (sqrt(v[1]) + sqrt(v[3])) / (1.0 + sqrt((double)n));
+  const double err_d = (sqrt(v[4]) + sqrt(v[0])) / (1.0 + nd.normL);
+  const double primal = v[2];
+  const double dual = v[6] + v[5];
+  const double gap = fabs(dual - primal) / (1.0 + fabs(dual) + fabs(primal));
+  const double err = err_p > err_d ? (err_p > gap ? err_p : gap) : (err_d > gap ? err_d : gap);
+  const double dgp = log10(err_p), dgd = log10(err_d);
+  if (dgd - dgp < -0.5)
+    nd.sigma = sg / nd.scaleSigma;
+  else if (dgd - dgp > 0.5)
+    nd.sigma = sg * nd.scaleSigma;
+  nd.err_p = err_p;
+  nd.err_d = err_d;
+  nd.gap = gap;
+  nd.err = err;
+  nd.primal = primal;
+  nd.dual = dual;
+  nd.nbit += 1;
+  if (err < nd.tol || nd.nbit > nd.itermax) nd.done = 1;
 }
 
 // f = n * lambda_max(L - Diag(y) - B^T(s)) + sum y + sum s (admm.c:757-763),
@@ -951,12 +978,13 @@ static int chunks_for(int work, int per) {
 }
 
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
-  dim3 grid(chunks_for(mmax, WGS * 4), nb);
+  // one cut per thread: the per-cut gathers are latency chains
+  dim3 grid(chunks_for(mmax, WGS), nb);
   k_admm_prep<<<grid, WGS, 0, st>>>(nodes);
 }
 
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, int maxit,
-               size_t lds_limit, hipStream_t st) {
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, double adapt,
+               int maxit, size_t lds_limit, hipStream_t st) {
   const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
   const bool i16 = mmax < 32768 && namax < 32768;
   // index streams staged in LDS as well: aptr (ints) and aent, cact
@@ -966,13 +994,13 @@ void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double r
 #define MADAM_CG1(CPT)                                                              \
   do {                                                                              \
     if (i16 && st_ok)                                                               \
-      k_cg1<CPT, short, true><<<nb, WG, lds_st, st>>>(nodes, rtol, maxit);          \
+      k_cg1<CPT, short, true><<<nb, WG, lds_st, st>>>(nodes, rtol, adapt, maxit);          \
     else if (i16)                                                                   \
-      k_cg1<CPT, short, false><<<nb, WG, lds, st>>>(nodes, rtol, maxit);            \
+      k_cg1<CPT, short, false><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);            \
     else if (st_ok)                                                                 \
-      k_cg1<CPT, int, true><<<nb, WG, lds_st, st>>>(nodes, rtol, maxit);            \
+      k_cg1<CPT, int, true><<<nb, WG, lds_st, st>>>(nodes, rtol, adapt, maxit);            \
     else                                                                            \
-      k_cg1<CPT, int, false><<<nb, WG, lds, st>>>(nodes, rtol, maxit);              \
+      k_cg1<CPT, int, false><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);              \
   } while (0)
   if (lds <= lds_limit && mmax <= 2 * WG) {
     MADAM_CG1(2);
@@ -980,18 +1008,19 @@ void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double r
     MADAM_CG1(4);
   } else if (lds <= lds_limit) {
     if (i16)
-      k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+      k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);
     else
-      k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, maxit);
+      k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);
   } else {
-    k_cg<<<nb, WG, 0, st>>>(nodes, rtol, maxit);
+    k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit);
   }
 #undef MADAM_CG1
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {
   int work = nmax * nmax / 2 > mmax ? nmax * nmax / 2 : mmax;
-  dim3 grid(chunks_for(work, WGS * 4), nb);
+  // two lower entries per thread: each is a chain of dependent gathers
+  dim3 grid(chunks_for(work, 2 * WGS), nb);
   k_build_w<<<grid, WGS, 0, st>>>(nodes, use_s);
 }
 
@@ -1018,9 +1047,12 @@ void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_s
   }
 }
 
-void launch_admm_post(NodeView* nodes, int nb, int nmax, hipStream_t st) {
-  (void)nmax;
-  k_admm_post<<<nb, WG, 0, st>>>(nodes);
+void launch_admm_post(NodeView* nodes, int nb, int nmax, int mmax, double* part,
+                      hipStream_t st) {
+  const int work = nmax * nmax / 2 > mmax ? nmax * nmax / 2 : mmax;
+  const int G = chunks_for(work, 2 * WGS);  // <= 64
+  k_post_part<<<dim3(G, nb), WGS, 0, st>>>(nodes, part);
+  k_post_fin<<<nb, 64, 0, st>>>(nodes, part, G);
 }
 
 void launch_safe_bound(NodeView* nodes, int nb, hipStream_t st) {

@@ -315,6 +315,7 @@ struct Engine {
   std::vector<Slot> slots;
   DBuf<NodeView> views;
   DBuf<double> tmpd;
+  DBuf<double> postpart;  // k_admm_post partial sums: nodes x 64 workgroups x 8
   std::vector<NodeView> hviews;
 };
 
@@ -347,6 +348,7 @@ inline madam_opts default_opts() {
   o.check_every = 16;
   o.projection = 0;
   o.ns_layout = 0;
+  o.cg_adapt = 0.0;
   return o;
 }
 

@@ -38,6 +38,7 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
   const int k = (int)hv.size();
   int rc;
   if ((rc = E.views.ensure(k))) return rc;
+  if ((rc = E.postpart.ensure((size_t)k * 64 * 8))) return rc;
   if ((rc = up(E.views.p, hv.data(), k, st))) return rc;
   const int chunk = std::max(1, o.check_every);
   int itmax_all = 0, namax = 1, nemax = 1;
@@ -71,7 +72,8 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
-            madam::launch_cg(E.views.p, k, mmax, namax, nemax, o.cg_rtol, o.cg_maxit, E.lds_limit, st));
+            madam::launch_cg(E.views.p, k, mmax, namax, nemax, o.cg_rtol, o.cg_adapt, o.cg_maxit,
+                             E.lds_limit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
         TIMED(PH_EIG, st, {
@@ -85,7 +87,7 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
         TIMED(PH_EIG, st,
               madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1));
       }
-      TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, st));
+      TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, mmax, E.postpart.p, st));
     }
     HIP_TRY(hipGetLastError());
     if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
@@ -184,6 +186,7 @@ void madam_close(void) {
   g_eng->nvals.release();
   g_eng->views.release();
   g_eng->tmpd.release();
+  g_eng->postpart.release();
   (void)hipStreamDestroy(g_eng->stream);
   delete g_eng;
   g_eng = nullptr;

@@ -69,13 +69,15 @@ struct NodeView {
 // Launch helpers implemented in madam_device.hip.  All take the device array
 // of nodes and the batch size, and run on `stream`.
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, int maxit,
-               size_t lds_limit, hipStream_t stream);
+void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, double adapt,
+               int maxit, size_t lds_limit, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
                    int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
 size_t jacobi_lds_bytes(int nmax, bool with_v);
-void launch_admm_post(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+// part: nb x 64 x 8 doubles of scratch for the per-workgroup partial sums
+void launch_admm_post(NodeView* nodes, int nb, int nmax, int mmax, double* part,
+                      hipStream_t stream);
 void launch_safe_bound(NodeView* nodes, int nb, hipStream_t stream);
 void launch_op_B(const NodeView* nodes, int nb, const double* Xin, double* yout, int mmax,
                  hipStream_t stream);

@@ -57,6 +57,7 @@ double SDPbound(BabNode *node, Problem *SP, Problem *PP, int rank) {
     c.stream_ctr = stream;
     c.projection = projection;
     c.cg_rtol = projection ? 1e-7 : 1e-12;
+    c.cg_adapt = projection ? 0.01 : 0.0;   /* inexact cut solves while ADMM is far off */
     c.batch = 1;
     madam_node *nodes[1] = {(madam_node *)node};
     int rk = rank;
