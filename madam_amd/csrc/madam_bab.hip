@@ -14,6 +14,7 @@
 
 #include <chrono>
 #include <memory>
+#include <unordered_map>
 #include <unordered_set>
 
 #include "madam_engine.h"
@@ -130,14 +131,15 @@ void store_fracsol(Run& r, int N, const std::vector<double>& lastcol) {
 
 int fetch_fracsol(std::vector<Run*>& runs, int N) {
   Engine& E = *g_eng;
-  std::vector<double> col;
-  for (Run* r : runs) {
-    col.resize(r->n);
-    int rc = down(col.data(), E.slots[r->slot].X.p + (size_t)(r->n - 1) * r->n, r->n, E.stream);
+  std::vector<std::vector<double>> cols(runs.size());
+  for (size_t q = 0; q < runs.size(); ++q) {
+    Run* r = runs[q];
+    cols[q].resize(r->n);
+    int rc = down(cols[q].data(), E.slots[r->slot].X.p + (size_t)(r->n - 1) * r->n, r->n, E.stream);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(E.stream));
-    store_fracsol(*r, N, col);
   }
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  for (size_t q = 0; q < runs.size(); ++q) store_fracsol(*runs[q], N, cols[q]);
   return MADAM_OK;
 }
 
@@ -173,15 +175,21 @@ int heuristic(madam_bab_ctx* c, std::vector<Run*>& runs, bool zero_planes) {
   if ((rc = up(E.views.p, hv.data(), k, st)) || (rc = up(E.gwnodes.p, g.data(), k, st))) return rc;
   TIMED(PH_GW, st, madam::launch_gw(E.views.p, E.gwnodes.p, k, nmax, num, st));
   HIP_TRY(hipGetLastError());
-  std::vector<signed char> z;
+  // every run's best value and cut in one round trip
+  std::vector<double> bvs(k);
+  std::vector<std::vector<signed char>> zs(k);
+  for (int q = 0; q < k; ++q) {
+    Slot& sl = E.slots[runs[q]->slot];
+    zs[q].resize(runs[q]->n);
+    if ((rc = down(&bvs[q], sl.gwbest.p, 1, st)) || (rc = down(zs[q].data(), sl.gwbz.p, runs[q]->n, st)))
+      return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
   std::vector<int> sol(c->N);
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
-    Slot& sl = E.slots[r.slot];
-    double bv = 0;
-    z.resize(r.n);
-    if ((rc = down(&bv, sl.gwbest.p, 1, st)) || (rc = down(z.data(), sl.gwbz.p, r.n, st))) return rc;
-    HIP_TRY(hipStreamSynchronize(st));
+    const double bv = bvs[q];
+    const std::vector<signed char>& z = zs[q];
     int idx = 0;
     for (int i = 0; i < c->N - 1; ++i)
       sol[i] = r.node->xfixed[i] ? r.node->solX[i] : (z[idx++] + 1) / 2;
@@ -234,6 +242,19 @@ void kgon_list(const std::vector<double>& cost, const std::vector<int>& perms,
   }
 }
 
+// FNV-1a over a pentagonal/heptagonal cut's type and vertex tuple.
+template <class Cut>
+uint64_t kgon_hash(const Cut& c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint32_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  mix((uint32_t)c.type);
+  for (size_t a = 0; a < sizeof(c.perm) / sizeof(c.perm[0]); ++a) mix((uint32_t)c.perm[a]);
+  return h;
+}
+
 // Purge (dual < 1e-5) and append non-duplicates (cutting_planes.c:132-214,
 // 316-409, 512-605).  Duplicates are checked against the survivors only.
 template <class Cut, class Same>
@@ -248,10 +269,16 @@ void purge_append(std::vector<Cut>& cuts, const double* dual, const std::vector<
     keep.push_back(x);
   }
   const size_t survivors = keep.size();
+  // survivors indexed by a hash of (type, perm); equal hashes are compared
+  // in full, so the outcome is the reference's linear scan
+  std::unordered_multimap<uint64_t, size_t> index;
+  index.reserve(survivors * 2);
+  for (size_t c = 0; c < survivors; ++c) index.emplace(kgon_hash(keep[c]), c);
   for (const Cut& cand : list) {
     if (keep.size() == 50000) break;
     bool dup = false;
-    for (size_t c = 0; c < survivors && !dup; ++c) dup = same(keep[c], cand);
+    const auto range = index.equal_range(kgon_hash(cand));
+    for (auto it = range.first; it != range.second && !dup; ++it) dup = same(keep[it->second], cand);
     if (dup) continue;
     Cut x = cand;
     x.y = 0.0;

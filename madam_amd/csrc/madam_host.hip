@@ -451,20 +451,30 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
   HIP_TRY(hipGetLastError());
   lists.assign(k, {});
   maxval.assign(k, 0.0);
-  std::vector<uint32_t> ctl(madam::CTL_N), keys;
-  std::vector<double> vals;
+  // two round trips for the whole batch: the counts, then the lists
+  std::vector<uint32_t> ctl((size_t)k * madam::CTL_N);
+  std::vector<unsigned long long> mbs(k);
   for (int b = 0; b < k; ++b) {
     SepSlot& sp = E.sep[b];
-    unsigned long long mb = 0;
-    if ((rc = down(ctl.data(), sp.ctl.p, madam::CTL_N, st)) || (rc = down(&mb, sp.maxbits.p, 1, st)))
+    if ((rc = down(ctl.data() + (size_t)b * madam::CTL_N, sp.ctl.p, madam::CTL_N, st)) ||
+        (rc = down(&mbs[b], sp.maxbits.p, 1, st)))
       return rc;
-    HIP_TRY(hipStreamSynchronize(st));
-    const uint32_t nf = ctl[madam::CTL_NFINAL];
-    keys.resize(nf);
-    vals.resize(nf);
-    if ((rc = down(keys.data(), sp.fkey.p, nf, st)) || (rc = down(vals.data(), sp.fval.p, nf, st)))
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<std::vector<uint32_t>> keys(k);
+  std::vector<std::vector<double>> vals(k);
+  for (int b = 0; b < k; ++b) {
+    SepSlot& sp = E.sep[b];
+    const uint32_t nf = ctl[(size_t)b * madam::CTL_N + madam::CTL_NFINAL];
+    keys[b].resize(nf);
+    vals[b].resize(nf);
+    if ((rc = down(keys[b].data(), sp.fkey.p, nf, st)) || (rc = down(vals[b].data(), sp.fval.p, nf, st)))
       return rc;
-    HIP_TRY(hipStreamSynchronize(st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int b = 0; b < k; ++b) {
+    const unsigned long long mb = mbs[b];
+    const uint32_t nf = (uint32_t)keys[b].size();
     const unsigned long long bits =
         (mb & 0x8000000000000000ull) ? (mb & 0x7fffffffffffffffull) : ~mb;
     double mx;
@@ -473,12 +483,12 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
     auto& L = lists[b];
     L.resize(nf);
     for (uint32_t q = 0; q < nf; ++q) {
-      const uint32_t key = keys[q];
+      const uint32_t key = keys[b][q];
       L[q].type = (int)(key >> 30) + 1;
       L[q].i = (key >> 20) & 1023;
       L[q].j = (key >> 10) & 1023;
       L[q].k = key & 1023;
-      L[q].value = vals[q];
+      L[q].value = vals[b][q];
       L[q].y = 0.0;
     }
   }
