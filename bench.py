@@ -89,7 +89,7 @@ def wl_c2(args):
         B = G.Bounder(SPL, G.default_params(), **(CHOSEN if form == "chosen" else PLAIN))
         b = B.sdpbound([B.node()], ranks=[0])[0]
         return {"bound": float(b), "bounds": np.array([b]), "admm_iters": int(B.ctx.admm_iters),
-                "nodes": 1, "n3_iters": float(B.ctx.admm_iters) * n ** 3}
+                "nodes": 1, "n3_iters": float(B.ctx.admm_iters) * n ** 3, "n": n}
 
     def cpu(seconds):
         from oracle import oracle_py as O
@@ -298,7 +298,11 @@ def main():
                     help="bab150: nodes per step (0: solve to the proven optimum)")
     ap.add_argument("--plain-nodes", type=int, default=4,
                     help="c3-batch: nodes bounded one at a time by the plain form")
+    ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2],
+                    help="chosen form's sign-iteration layout: 0 automatic, 1 one workgroup "
+                         "per node, 2 tile-parallel")
     args = ap.parse_args()
+    CHOSEN["ns_layout"] = args.ns_layout
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the bounding path "
               "shards by B&B node across GPUs, which this build does not run.", file=sys.stderr)
@@ -335,18 +339,26 @@ def main():
         sec = eig_ms * 1e-3
         achieved = (f32 + f64) / sec / 1e12 if sec > 0 else 0.0
         t_peak = (f32 / PEAK_FP32_MFMA + f64 / PEAK_FP64_MFMA) / 1e12
-        # HBM bytes per projection of k_ns_project, measured for this workload
-        # with rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
-        # (fetch x2.00, write x1.00): 335.9 + 234.4 KiB
-        traffic = 570.3 * 1024 if args.workload == "c2-root" else None
-        roofline = {"bound": "mfma", "kernel": "k_ns_project (sign-iteration projection: 39 FP32-MFMA "
-                    "products + 3 FP64-MFMA products)", "achieved": achieved,
+        # HBM bytes per projection, measured for this workload with rocprofv3
+        # FETCH_SIZE / WRITE_SIZE (per-dispatch KiB scaled by a 64 MiB copy:
+        # fetch x2.00, write x1.00), summed over the launches of one
+        # projection (one node of order 100: tile-parallel layout, 39 x
+        # k_ns_gemm32 135.5 KiB + 3 x k_ns_gemm64 213.6 + k_ns_init 123.5 +
+        # k_ns_conv 132.4) and scaled to the projections that did work
+        # (798 of the 864 timed launches): the FP32 iterates pass through L2
+        # between launches.  Algorithmic: W in, X+ out, 2 x 8 n^2 bytes.
+        traffic = 6692.0 * 1024 if args.workload == "c2-root" else None
+        roofline = {"bound": "mfma", "kernel": "sign-iteration projection (39 FP32-MFMA products + 3 "
+                    "FP64-MFMA products; k_ns_gemm32/k_ns_gemm64 tile launches or k_ns_project)",
+                    "achieved": achieved,
                     "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
                     "frac": t_peak / sec if sec > 0 else 0.0, "traffic": traffic,
                     "frac_note": "time at the FP32/FP64 MFMA peaks for these flops / measured time",
                     "launches": launches, "avg_launch_ms": eig_ms / max(1, launches),
                     "ms_per_projection": eig_ms / max(1, rp["admm_iters"]),
                     "flops_per_projection": "84 n^3 (78 n^3 FP32 + 6 n^3 FP64)",
+                    "traffic_unit": "HBM bytes per projection",
+                    "algorithmic_bytes_per_projection": 16.0 * rp.get("n", 0) ** 2 or None,
                     "projections": rp["admm_iters"],
                     "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
 
