@@ -279,8 +279,39 @@ def wl_bab150(args):
     return cfg, step, cpu
 
 
+def wl_ns_batch(args):
+    """Throughput of the projection kernels in a full batch: k copies of
+    one node of order n (G(n,0.3) objective, triangle-free basic SDP, X = I
+    start), 64 ADMM iterations each, one ADMM_solver_batch call per step.
+    The bound of each node is an ordinary ADMM safe bound; the line is
+    about the eig phase (roofline)."""
+    import madam_amd as G
+    from madam_amd import instances as I
+    n, k = args.ns_n, args.ns_k
+    vals = [v for v in range(-10, 11) if v != 0]
+    L = I.objective_matrix(*I.gnp(n, 0.3, vals, 5))
+
+    def step(form, sample=None):
+        kk = k if form == "chosen" else min(k, 2)
+        probs = [dict(L=L, X=np.eye(L.shape[0]), y=np.diag(L).copy(), Q=np.zeros_like(L),
+                      sigma=1.0, tol=1e-14, itermax=64) for _ in range(kk)]
+        opts = dict(projection=1, cg_rtol=1e-7) if form == "chosen" else dict(projection=0)
+        res = G.ADMM_solver_batch(probs, **opts)
+        it = sum(r.nbit for r in res)
+        bs = np.array([r.f for r in res])
+        return {"bound": float(bs[0]), "bounds": bs, "admm_iters": it, "nodes": kk,
+                "n3_iters": float(it) * L.shape[0] ** 3}
+
+    def cpu(seconds):
+        return None, "not timed for this workload", None
+
+    cfg = {"workload": f"projection throughput: {k} nodes of order {n} in one batch, "
+                       "64 ADMM iterations each", "n": n, "nodes_per_step": k}
+    return cfg, step, cpu
+
+
 WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4,
-             "bab150": wl_bab150}
+             "bab150": wl_bab150, "ns-batch": wl_ns_batch}
 
 
 def main():
@@ -298,9 +329,14 @@ def main():
                     help="bab150: nodes per step (0: solve to the proven optimum)")
     ap.add_argument("--plain-nodes", type=int, default=4,
                     help="c3-batch: nodes bounded one at a time by the plain form")
+    ap.add_argument("--ns-n", type=int, default=150, help="ns-batch: node order")
+    ap.add_argument("--ns-k", type=int, default=128, help="ns-batch: nodes in the batch")
     ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2],
                     help="chosen form's sign-iteration layout: 0 automatic, 1 one workgroup "
                          "per node, 2 tile-parallel")
+    ap.add_argument("--timed-form", default="chosen", choices=["chosen", "plain"],
+                    help="form run in the warmup, timed and profiled steps (plain: to profile "
+                         "the plain FP64 form alone; the line then reports it as value)")
     args = ap.parse_args()
     CHOSEN["ns_layout"] = args.ns_layout
     if args.gpus != 1:
@@ -314,12 +350,12 @@ def main():
     cfg, step, cpu = WORKLOADS[args.workload](args)
 
     for w in range(args.warmup):
-        step("chosen")
+        step(args.timed_form)
         log(f"warmup step {w + 1} of {args.warmup} done")
     t0 = time.perf_counter()
     nodes = 0
     for k in range(args.steps):
-        r = step("chosen")
+        r = step(args.timed_form)
         nodes += r["nodes"]
         log(f"timed step {k + 1} of {args.steps} done ({time.perf_counter() - t0:.2f} s)")
     dt = time.perf_counter() - t0
@@ -327,9 +363,9 @@ def main():
     value = nodes / dt
 
     roofline = None
-    if not args.no_profile:
+    if not args.no_profile and args.timed_form == "chosen":
         with Timer(lib) as tm:
-            rp = step("chosen")
+            rp = step(args.timed_form)
         log("profiled step done")
         eig = PHASES.index("eig")
         eig_ms = float(tm.ms[eig])
