@@ -255,10 +255,15 @@ __device__ __forceinline__ void sym_gemm_f64(const double* __restrict__ A, const
   for (int t = wave; t < ntile; t += nw) gemm_tile_f64(A, B, Cm, n, alpha, beta, t);
 }
 
-// Projection of nd.W (FP64) -> X_+ in nd.S1, one workgroup per node.  The
-// three FP32 iterates live in LDS when they fit (3 n_p^2 floats, n_p <= 112:
-// operand reads at LDS latency), else in the node's ns32 scratch.
-template <bool LDS>
+// Projection of nd.W (FP64) -> X_+ in nd.S1, one workgroup per node.
+// MODE 1: the three FP32 iterates live in LDS (3 n_p^2 floats, n_p <=
+// 112); MODE 2: one n_p^2 LDS buffer (n_p <= 192) holding the matrix both
+// operands of the next product read -- S for T = S^2, T for U, U for S' --
+// refilled from the node's ns32 scratch by a coalesced copy after each
+// product, so only S' = S (a I + U) reads one operand from L2/HBM; MODE 0:
+// all three in ns32.  The products and their bits are the same in every
+// mode.
+template <int MODE>
 __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore_done) {
   extern __shared__ float sm32[];
   __shared__ double red[64];
@@ -266,9 +271,10 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   if (!ignore_done && nd.done) return;
   const int n = nd.n, np = (n + 15) & ~15, tid = threadIdx.x;
   const size_t npp = (size_t)np * np;
-  float* Sb = LDS ? sm32 : nd.ns32;
+  float* Sb = MODE == 1 ? sm32 : nd.ns32;
   float* Tb = Sb + npp;
   float* Ub = Tb + npp;
+  float* Lb = sm32;  // MODE 2 staging buffer
   const double* __restrict__ Wm = nd.W;
   double fv[1] = {0.0};
   for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
@@ -277,20 +283,41 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   const float inv = fro > 0.0 ? (float)(1.0 / fro) : 0.f;
   for (size_t f = tid; f < npp; f += WG) {
     const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
-    Sb[f] = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
+    const float v = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
+    Sb[f] = v;
+    if (MODE == 2) Lb[f] = v;
   }
   __syncthreads();
+  // global -> LDS copy of one n_p x n_p matrix (n_p^2 is a multiple of 256)
+  auto stage = [&](const float* G) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    for (size_t f = (size_t)tid * 4; f < npp; f += (size_t)WG * 4)
+      *(f4*)(Lb + f) = *(const f4*)(G + f);
+    __syncthreads();
+  };
   float* S = Sb;
   float* T = Tb;
   float* U = Ub;
   for (int s = 0; s < NS_STEPS; ++s) {
     const float a = NS_COEF[s][0], b = NS_COEF[s][1], c = NS_COEF[s][2];
-    sym_gemm_f32(S, S, T, np, 0.f, 1.f);  // T = S^2
-    __syncthreads();
-    sym_gemm_f32(T, T, U, np, b, c);      // U = T (b I + c T)
-    __syncthreads();
-    sym_gemm_f32(S, U, T, np, a, 1.f);    // S' = S (a I + U), into T
-    __syncthreads();
+    if (MODE == 2) {
+      sym_gemm_f32(Lb, Lb, T, np, 0.f, 1.f);  // T = S^2 (S staged)
+      __syncthreads();
+      stage(T);
+      sym_gemm_f32(Lb, Lb, U, np, b, c);      // U = T (b I + c T)
+      __syncthreads();
+      stage(U);
+      sym_gemm_f32(S, Lb, T, np, a, 1.f);     // S' = S (a I + U), into T
+      __syncthreads();
+      stage(T);
+    } else {
+      sym_gemm_f32(S, S, T, np, 0.f, 1.f);  // T = S^2
+      __syncthreads();
+      sym_gemm_f32(T, T, U, np, b, c);      // U = T (b I + c T)
+      __syncthreads();
+      sym_gemm_f32(S, U, T, np, a, 1.f);    // S' = S (a I + U), into T
+      __syncthreads();
+    }
     float* tmp = S;
     S = T;
     T = tmp;
@@ -334,9 +361,11 @@ void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_
                        hipStream_t st) {
   const size_t lds = ns_lds_bytes(nmax);
   if (lds + 1024 <= lds_limit)
-    k_ns_project<true><<<nb, 1024, lds, st>>>(nodes, ignore_done);
+    k_ns_project<1><<<nb, 1024, lds, st>>>(nodes, ignore_done);
+  else if (lds / 3 + 1024 <= lds_limit)
+    k_ns_project<2><<<nb, 1024, lds / 3, st>>>(nodes, ignore_done);
   else
-    k_ns_project<false><<<nb, 1024, 0, st>>>(nodes, ignore_done);
+    k_ns_project<0><<<nb, 1024, 0, st>>>(nodes, ignore_done);
 }
 
 // ---------------------------------------------------------------------------

@@ -109,12 +109,22 @@ def test_sdpbound_root_ns_stable_across_cut_solves(cg_rtol):
     assert B.ctx.admm_iters < 2000
 
 
-@pytest.mark.parametrize("graph,kgon", [("g05_60.0", False), ("w01_100.3", True)])
+def _layout_graph(spec):
+    if spec.startswith("gnp"):  # G(n, 0.3), weights +-{1..10}: orders past the LDS-resident kernel
+        n = int(spec[3:])
+        return I.gnp(n, 0.3, [v for v in range(-10, 11) if v != 0], 11)
+    return spec
+
+
+# orders 60 and 100: one-workgroup kernel with the iterates in LDS; 150: one
+# staged LDS buffer; 200: iterates in HBM
+@pytest.mark.parametrize("graph,kgon", [("g05_60.0", False), ("w01_100.3", True),
+                                        ("gnp150", False), ("gnp200", False)])
 def test_tile_parallel_layout_is_bit_identical(graph, kgon):
     """The tile-parallel launch sequence computes the same tiles with the
-    same instruction order as the one-workgroup-per-node kernel: the ADMM
-    results must agree bit for bit."""
-    sol, case = admm_case(graph_L(graph), kgon=kgon)
+    same instruction order as the one-workgroup-per-node kernel in each of
+    its storage modes: the ADMM results must agree bit for bit."""
+    sol, case = admm_case(graph_L(_layout_graph(graph)), kgon=kgon)
     case["tol"] = 1e-3
     a = G.ADMM_solver(**case, projection=1, ns_layout=1)
     b = G.ADMM_solver(**case, projection=1, ns_layout=2)
