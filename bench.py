@@ -231,7 +231,7 @@ def wl_c4(args):
 def wl_bab150(args):
     """Branch and bound of pw05_150.0 (rudy, n=150; the reference recorded
     optimum 17852, root bound 18083.50, 66941 nodes in 576 s on 240 cores).
-    Chosen: node pipeline of 256 slots; a step bounds args.bab_nodes nodes
+    Chosen: node pipeline of 1024 slots; a step bounds args.bab_nodes nodes
     (0: the whole tree, to the proven optimum).  Plain: one node at a time,
     FP64 Jacobi, on the first 200 nodes, and beside it the chosen form one
     node at a time on the same 200 nodes (the projection and cut-solve gain
@@ -245,7 +245,7 @@ def wl_bab150(args):
 
     def step(form, sample=None):
         if form == "chosen":
-            B = G.Bounder(SPL, G.default_params(), batch=256, verbose=1, **CHOSEN)
+            B = G.Bounder(SPL, G.default_params(), batch=1024, verbose=1, **CHOSEN)
             opt = B.bab(max_nodes=args.bab_nodes)
         else:
             B1 = G.Bounder(SPL, G.default_params(), batch=1, **CHOSEN)
@@ -273,7 +273,7 @@ def wl_bab150(args):
         dt = time.perf_counter() - t0
         return s.s.nodes / dt, f"oracle B&B of pw05_150.0, first {s.s.nodes} nodes in {dt:.1f} s", None
 
-    cfg = {"workload": "branch and bound of pw05_150.0 (n=150, rudy), node pipeline of 256, "
+    cfg = {"workload": "branch and bound of pw05_150.0 (n=150, rudy), node pipeline of 1024, "
                        + ("to the proven optimum" if args.bab_nodes <= 0 else f"first {args.bab_nodes} nodes"),
            "n": 150, "nodes_per_step": "B&B"}
     return cfg, step, cpu

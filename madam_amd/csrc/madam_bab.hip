@@ -296,7 +296,7 @@ int separate_tri(madam_bab_ctx* c, std::vector<Run*>& runs) {
   std::vector<double> mx;
   int rc = tri_separate_slots(slots, K.data(), c->params.violated_TriIneq, lists, mx);
   if (rc) return rc;
-  for (int q = 0; q < k; ++q) {
+  parallel_for(k, [&](int q) {
     Run& r = *runs[q];
     std::vector<double> dual(r.tri.size());
     for (size_t i = 0; i < r.tri.size(); ++i) dual[i] = r.s[i];
@@ -321,7 +321,7 @@ int separate_tri(madam_bab_ctx* c, std::vector<Run*>& runs) {
     }
     r.tri.swap(keep);
     r.viol3 = mx[q];
-  }
+  });
   return MADAM_OK;
 }
 
@@ -373,9 +373,9 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
   if ((rc = down(out.data(), E.saout.p, total, st))) return rc;
   HIP_TRY(hipStreamSynchronize(st));
   E.timer.flush();
-  std::vector<double> cost(per);
-  std::vector<int> perms((size_t)per * KK);
-  for (int q = 0; q < k; ++q) {
+  parallel_for(k, [&](int q) {
+    std::vector<double> cost(per);
+    std::vector<int> perms((size_t)per * KK);
     Run& R = *runs[q];
     for (int r = 0; r < per; ++r) {
       const madam::SaOut& o = out[(size_t)q * per + r];
@@ -404,7 +404,7 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
       });
       R.viol5 = best;
     }
-  }
+  });
   return MADAM_OK;
 }
 
@@ -418,9 +418,22 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   std::vector<CutSet> cs(k);
   std::vector<NodeView> hv(k);
   int nmax = 2, mmax = 1;
+  // cut structures on host threads; a node whose cuts are invalid is built
+  // again on this thread so that its error message is set
+  std::vector<char> bad(k, 0);
+  parallel_for(k, [&](int q) {
+    Run& r = *runs[q];
+    if (!cuts_valid(r.n, r.tri, r.pent, r.hept)) {
+      bad[q] = 1;
+      return;
+    }
+    build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
+                 (int)r.hept.size(), r.hept.data(), cs[q]);
+  });
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
-    if ((rc = build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
+    if (bad[q] &&
+        (rc = build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
                            (int)r.hept.size(), r.hept.data(), cs[q])))
       return rc;
     Slot& sl = E.slots[r.slot];

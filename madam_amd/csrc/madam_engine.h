@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/madam_bound.h"
@@ -69,6 +70,29 @@ inline int kgon_expand(int k, int type, const int* perm, int* v, int* b) {
     b[a] = (a < type - 1) ? -1 : 1;
   }
   return k;
+}
+
+// The checks build_cutset reports as errors, without touching shared
+// state (callable from several threads): every vertex in 0..n-1, no vertex
+// twice in one cut.
+inline bool cuts_valid(int n, const std::vector<madam_tri>& tri, const std::vector<madam_pent>& pent,
+                       const std::vector<madam_hept>& hept) {
+  int v[7], b[7];
+  auto ok = [&](int k) {
+    for (int a = 0; a < k; ++a) {
+      if (v[a] < 0 || v[a] >= n) return false;
+      for (int q = 0; q < a; ++q)
+        if (v[q] == v[a]) return false;
+    }
+    return true;
+  };
+  for (const auto& c : tri)
+    if (!ok(tri_expand(c, v, b))) return false;
+  for (const auto& c : pent)
+    if (!ok(kgon_expand(5, c.type, c.perm, v, b))) return false;
+  for (const auto& c : hept)
+    if (!ok(kgon_expand(7, c.type, c.perm, v, b))) return false;
+  return true;
 }
 
 inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const madam_pent* pent,
@@ -155,6 +179,27 @@ inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const 
     cs.cact16.assign(cs.cact.begin(), cs.cact.end());
   }
   return MADAM_OK;
+}
+
+// f(i) for i in [0, n) on up to 8 host threads (static chunks); the
+// per-node host work of a batch (cut structures, separation lists) is
+// independent across nodes.  f must not touch shared state.
+template <class F>
+inline void parallel_for(int n, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = std::min<int>(std::min<unsigned>(hw ? hw : 1, 8u), n / 8);
+  if (T <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(T - 1);
+  auto run = [&](int t) {
+    for (int i = (int)((long)n * t / T); i < (int)((long)n * (t + 1) / T); ++i) f(i);
+  };
+  for (int t = 1; t < T; ++t) pool.emplace_back(run, t);
+  run(0);
+  for (auto& th : pool) th.join();
 }
 
 // ---------------------------------------------------------------- engine
@@ -352,10 +397,42 @@ inline madam_opts default_opts() {
   return o;
 }
 
+// Host-to-device uploads go through a pinned ring buffer: the source is
+// copied into it at once (so the caller may reuse it, as with a pageable
+// copy) and the DMA runs asynchronously on the stream.  When the ring is
+// full the stream is synchronised before its start is reused; uploads of
+// more than a quarter of the ring go directly from pageable memory.
+struct PinnedRing {
+  char* p = nullptr;
+  size_t cap = 0, head = 0;
+  static constexpr size_t kBytes = 64u << 20;
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = head = 0;
+  }
+};
+inline PinnedRing g_ring;
+
 template <class T>
 inline int up(T* dst, const T* src, size_t n, hipStream_t s) {
   if (n == 0) return MADAM_OK;
-  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  const size_t bytes = n * sizeof(T);
+  PinnedRing& R = g_ring;
+  if (!R.p && hipHostMalloc((void**)&R.p, PinnedRing::kBytes, hipHostMallocDefault) == hipSuccess)
+    R.cap = PinnedRing::kBytes;
+  if (!R.p || bytes > R.cap / 4) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return MADAM_OK;
+  }
+  size_t off = (R.head + 255) & ~(size_t)255;
+  if (off + bytes > R.cap) {
+    HIP_TRY(hipStreamSynchronize(s));  // every copy out of the ring has finished
+    off = 0;
+  }
+  memcpy(R.p + off, src, bytes);
+  HIP_TRY(hipMemcpyAsync(dst, R.p + off, bytes, hipMemcpyHostToDevice, s));
+  R.head = off + bytes;
   return MADAM_OK;
 }
 template <class T>

@@ -188,6 +188,7 @@ void madam_close(void) {
   g_eng->tmpd.release();
   g_eng->postpart.release();
   (void)hipStreamDestroy(g_eng->stream);
+  g_ring.release();
   delete g_eng;
   g_eng = nullptr;
 }
