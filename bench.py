@@ -48,7 +48,7 @@ METRIC = "time-to-proven-optimum + node bounds/sec, chosen vs plain FP64 GPU at 
 PEAK_FP32_MFMA = 157.3  # TFLOP/s, dense
 PEAK_FP64_MFMA = 78.6
 PHASES = ["prep", "cg", "w", "eig", "post", "safe", "ipm", "trisep", "sa", "gw", "zfac"]
-CHOSEN = dict(projection=1, cg_rtol=1e-7, cg_adapt=0.01)
+CHOSEN = dict(projection=1, cg_rtol=1e-7, cg_adapt=0.01, gw_mult=4)
 PLAIN = dict(projection=0, cg_rtol=1e-12)
 
 

@@ -235,6 +235,8 @@ typedef struct {
                                      * SDPbound calls would; LB ends as the best of them.
                                      * madam_bab ignores it (one shared incumbent). */
     double cg_adapt;                /* as madam_opts.cg_adapt */
+    int gw_mult;                    /* hyperplanes per rounding-heuristic call: gw_mult x N
+                                     * (<= 1: N, as heuristic.c) */
 } madam_bab_ctx;
 
 /* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n

@@ -375,7 +375,7 @@ class BabCtx(C.Structure):
         ("seconds_ipm", C.c_double), ("seconds_admm", C.c_double), ("seconds_sep", C.c_double),
         ("seconds_heur", C.c_double), ("cg_rtol", C.c_double), ("cg_iters", C.c_longlong),
         ("ns_fallbacks", C.c_longlong), ("verbose", C.c_int), ("ns_layout", C.c_int),
-        ("own_incumbent", C.c_int), ("cg_adapt", C.c_double),
+        ("own_incumbent", C.c_int), ("cg_adapt", C.c_double), ("gw_mult", C.c_int),
     ]
 
 
@@ -446,7 +446,7 @@ class Bounder:
     """SDPbound / branch and bound over one instance (SP->L) on the GPU."""
 
     def __init__(self, SPL, params=None, seed=2020, batch=1, LB=0.0, projection=0, cg_rtol=0.0,
-                 verbose=0, own_incumbent=0, ns_layout=0, cg_adapt=0.0):
+                 verbose=0, own_incumbent=0, ns_layout=0, cg_adapt=0.0, gw_mult=1):
         self.SPL = _f64(SPL)
         self.ctx = BabCtx()
         self.ctx.N = self.SPL.shape[0]
@@ -463,6 +463,7 @@ class Bounder:
         self.ctx.own_incumbent = own_incumbent
         self.ctx.ns_layout = ns_layout
         self.ctx.cg_adapt = cg_adapt
+        self.ctx.gw_mult = gw_mult
 
     def sdpbound(self, nodes, ranks=None):
         k = len(nodes)

@@ -151,7 +151,7 @@ int heuristic(madam_bab_ctx* c, std::vector<Run*>& runs, bool zero_planes) {
   const int k = (int)runs.size();
   if (k == 0) return MADAM_OK;
   int rc;
-  const int num = zero_planes ? 1 : c->N;
+  const int num = zero_planes ? 1 : c->N * (c->gw_mult > 1 ? c->gw_mult : 1);
   std::vector<madam::GwNode> g(k);
   std::vector<NodeView> hv(k);
   int nmax = 2;

@@ -58,6 +58,7 @@ double SDPbound(BabNode *node, Problem *SP, Problem *PP, int rank) {
     c.projection = projection;
     c.cg_rtol = projection ? 1e-7 : 1e-12;
     c.cg_adapt = projection ? 0.01 : 0.0;   /* inexact cut solves while ADMM is far off */
+    c.gw_mult = projection ? 4 : 1;          /* 4N rounding hyperplanes (heuristic.c: N) */
     c.batch = 1;
     madam_node *nodes[1] = {(madam_node *)node};
     int rk = rank;
