@@ -279,6 +279,39 @@ def wl_bab150(args):
     return cfg, step, cpu
 
 
+def wl_c5(args):
+    """Root SDPbound of the 8x8x8 toroidal grid (n=512, Gaussian weights
+    x100 rounded), tri+pent+hept cuts capped at 3000 + 1000 + 1000 per
+    round, chosen form; then --c5-nodes B&B nodes with a pipeline of 64.
+    The plain form (Jacobi on n = 512, thousands of ADMM iterations) and the
+    CPU oracle are not run: each would take many minutes per bound."""
+    import madam_amd as G
+    from madam_amd import instances as I
+    SPL = I.objective_matrix(*I.torus3d(8, seed=5))
+    params = dict(TriIneq=3000, adjust_TriIneq=0, PentIneq=1000, HeptaIneq=1000)
+
+    def step(form, sample=None):
+        B = G.Bounder(SPL, G.default_params(**params), verbose=1, **CHOSEN)
+        b = B.sdpbound([B.node()], ranks=[0])[0]
+        out = {"bound": float(b), "bounds": np.array([b]), "admm_iters": int(B.ctx.admm_iters),
+               "nodes": 1, "n3_iters": float(B.ctx.admm_iters) * 512 ** 3,
+               "incumbent": float(B.ctx.LB)}
+        if args.c5_nodes > 0:
+            t0 = time.perf_counter()
+            B2 = G.Bounder(SPL, G.default_params(**params), batch=64, **CHOSEN)
+            B2.bab(max_nodes=args.c5_nodes)
+            out["bab"] = {"nodes": int(B2.ctx.nodes), "seconds": time.perf_counter() - t0,
+                          "incumbent": float(B2.ctx.LB), "open": int(B2.ctx.open_nodes)}
+        return out
+
+    def cpu(seconds):
+        return None, "not timed for this workload", None
+
+    cfg = {"workload": "root SDPbound of the 8x8x8 torus (n=512), Gaussian weights x100, "
+                       "<= 5000 tri/pent/hept cuts per round", "n": 512, "nodes_per_step": 1}
+    return cfg, step, cpu
+
+
 def wl_ns_batch(args):
     """Throughput of the projection kernels in a full batch: k copies of
     one node of order n (G(n,0.3) objective, triangle-free basic SDP, X = I
@@ -311,7 +344,7 @@ def wl_ns_batch(args):
 
 
 WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4,
-             "bab150": wl_bab150, "ns-batch": wl_ns_batch}
+             "bab150": wl_bab150, "ns-batch": wl_ns_batch, "c5-root": wl_c5}
 
 
 def main():
@@ -329,6 +362,7 @@ def main():
                     help="bab150: nodes per step (0: solve to the proven optimum)")
     ap.add_argument("--plain-nodes", type=int, default=4,
                     help="c3-batch: nodes bounded one at a time by the plain form")
+    ap.add_argument("--c5-nodes", type=int, default=0, help="c5-root: B&B nodes after the root")
     ap.add_argument("--ns-n", type=int, default=150, help="ns-batch: node order")
     ap.add_argument("--ns-k", type=int, default=128, help="ns-batch: nodes in the batch")
     ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2],
@@ -399,7 +433,7 @@ def main():
                     "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
 
     plain = None
-    if not args.no_plain:
+    if not args.no_plain and args.workload != "c5-root":
         sample = args.plain_nodes if args.workload == "c3-batch" else None
         t1 = time.perf_counter()
         pr = step("plain", sample)
@@ -434,6 +468,9 @@ def main():
             "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
     if "admm_calls" in r:
         line["chosen"]["admm_calls"] = r["admm_calls"]
+    if args.workload == "c5-root":
+        line["c5"] = {"root_bound": r["bound"], "incumbent": r["incumbent"],
+                      "gap": (r["bound"] - r["incumbent"]) / r["incumbent"], "bab": r.get("bab")}
     if args.workload == "c1-bab" or (args.workload == "bab150" and r.get("open") == 0):
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]
