@@ -12,6 +12,7 @@
 """
 import itertools
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -19,7 +20,7 @@ import pytest
 import madam_amd as G
 from madam_amd import instances as I
 from oracle import oracle_py as O
-from tests.helpers import graph_L
+from tests.helpers import GOLDEN, graph_L
 
 pytestmark = pytest.mark.gpu
 
@@ -32,9 +33,14 @@ def brute_force(SPL):
     return vals.max()
 
 
-@pytest.mark.parametrize("spec", ["g05_60.0", "w01_100.3", "config2"])
+@pytest.mark.parametrize("spec", ["g05_60.0", "w01_100.3", "config2", "pw05_150.0"])
 def test_ipm_matches_oracle(spec):
-    SPL = graph_L(I.config2(2) if spec == "config2" else spec)
+    if spec == "config2":
+        SPL = graph_L(I.config2(2))
+    elif spec == "pw05_150.0":
+        SPL = graph_L(I.read_rudy(os.path.join(GOLDEN, "rudy150", spec)))
+    else:
+        SPL = graph_L(spec)
     L, _ = O.create_subproblem(SPL, np.zeros(SPL.shape[0] - 1, np.int32),
                                np.zeros(SPL.shape[0] - 1, np.int32))
     Xg, yg, Zg, phig, itg = G.ipm(L)
