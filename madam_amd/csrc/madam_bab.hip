@@ -408,16 +408,39 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
   return MADAM_OK;
 }
 
-// One round of ADMM_solver calls for the runs (admm.c:49-771).
-int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
+// ADMM calls in flight: their runs and device views, the views kept up to
+// date on the host between chunks of iterations.
+struct Flight {
+  std::vector<Run*> runs;
+  std::vector<NodeView> hv;
+  std::vector<int> m;  // cuts of each call
+};
+
+madam_opts bab_opts(const madam_bab_ctx* c) {
+  madam_opts o;
+  o.cg_rtol = c->cg_rtol > 0 ? c->cg_rtol : 1e-12;
+  o.cg_maxit = 1000;
+  o.jacobi_sweeps = 40;
+  o.check_every = 16;
+  o.projection = c->projection;
+  o.ns_layout = c->ns_layout;
+  o.cg_adapt = c->cg_adapt;
+  return o;
+}
+
+// Start one ADMM_solver call (admm.c:49-771) per run: its cut structure,
+// dual warm start and a fresh view join the calls in flight.
+int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
   Engine& E = *g_eng;
   const hipStream_t st = E.stream;
   const int k = (int)runs.size();
   if (k == 0) return MADAM_OK;
   int rc;
+  for (Run* r : runs) {
+    ++r->count;
+    r->oldf = r->f;
+  }
   std::vector<CutSet> cs(k);
-  std::vector<NodeView> hv(k);
-  int nmax = 2, mmax = 1;
   // cut structures on host threads; a node whose cuts are invalid is built
   // again on this thread so that its error message is set
   std::vector<char> bad(k, 0);
@@ -455,33 +478,64 @@ int admm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     fresh.tol = r.tol;
     fresh.scaleSigma = c->params.scaleSigma;
     fresh.itermax = c->params.ADMM_itermax;
-    hv[q] = fresh;
-    nmax = std::max(nmax, r.n);
-    mmax = std::max(mmax, cs[q].m);
+    F.runs.push_back(&r);
+    F.hv.push_back(fresh);
+    F.m.push_back(cs[q].m);
   }
-  madam_opts o;
-  o.cg_rtol = c->cg_rtol > 0 ? c->cg_rtol : 1e-12;
-  o.cg_maxit = 1000;
-  o.jacobi_sweeps = 40;
-  o.check_every = 16;
-  o.projection = c->projection;
-  o.ns_layout = c->ns_layout;
-  o.cg_adapt = c->cg_adapt;
-  if ((rc = admm_loop(hv, o, nmax, mmax))) return rc;
+  return MADAM_OK;
+}
+
+// Advance the calls in flight until at least min_done of them (all, with
+// min_done <= 0) have stopped; those leave the flight with their safe bound
+// (admm.c:718-763) and duals, listed in `done`.
+int advance_calls(madam_bab_ctx* c, Flight& F, int min_done, std::vector<Run*>& done) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  done.clear();
+  const int k = (int)F.runs.size();
+  if (k == 0) return MADAM_OK;
+  int rc, nmax = 2, mmax = 1;
   for (int q = 0; q < k; ++q) {
-    Run& r = *runs[q];
-    E.hviews[r.slot] = hv[q];
-    r.sigma = hv[q].sigma;
-    r.nbit = hv[q].nbit;
-    r.f = hv[q].f;
-    c->admm_iters += hv[q].nbit;
-    c->cg_iters += hv[q].cg_total;
-    c->ns_fallbacks += hv[q].ns_fallbacks;
+    nmax = std::max(nmax, F.runs[q]->n);
+    mmax = std::max(mmax, F.m[q]);
+  }
+  const madam_opts o = bab_opts(c);
+  if ((rc = admm_advance(F.hv, o, nmax, mmax, min_done >= k ? 0 : min_done))) return rc;
+  Flight rest;
+  std::vector<NodeView> hd;
+  std::vector<int> md;
+  int nd = 2, ndm = 1;
+  for (int q = 0; q < k; ++q) {
+    if (F.hv[q].done) {
+      done.push_back(F.runs[q]);
+      hd.push_back(F.hv[q]);
+      md.push_back(F.m[q]);
+      nd = std::max(nd, F.runs[q]->n);
+      ndm = std::max(ndm, F.m[q]);
+    } else {
+      rest.runs.push_back(F.runs[q]);
+      rest.hv.push_back(F.hv[q]);
+      rest.m.push_back(F.m[q]);
+    }
+  }
+  if ((rc = admm_finish(hd, o, nd, ndm))) return rc;
+  for (size_t q = 0; q < done.size(); ++q) {
+    Run& r = *done[q];
+    E.hviews[r.slot] = hd[q];
+    r.sigma = hd[q].sigma;
+    r.nbit = hd[q].nbit;
+    r.f = hd[q].f;
+    c->admm_iters += hd[q].nbit;
+    c->cg_iters += hd[q].cg_total;
+    c->ns_fallbacks += hd[q].ns_fallbacks;
     c->admm_calls += 1;
-    r.s.resize(cs[q].m);
-    if (cs[q].m > 0 && (rc = down(r.s.data(), E.slots[r.slot].s.p, cs[q].m, st))) return rc;
+    r.s.resize(md[q]);
+    if (md[q] > 0 && (rc = down(r.s.data(), E.slots[r.slot].s.p, md[q], st))) return rc;
   }
   HIP_TRY(hipStreamSynchronize(st));
+  F.runs.swap(rest.runs);
+  F.hv.swap(rest.hv);
+  F.m.swap(rest.m);
   return MADAM_OK;
 }
 
@@ -581,16 +635,13 @@ int start_runs(madam_bab_ctx* c, std::vector<Run*>& all) {
 // One round of the cutting-plane loop (bounding.c:130-236) for the active
 // runs: an ADMM call each, the heuristic, the give-up and prune tests and
 // the separation for the runs that go on; done is set for the others.
-int cut_round(madam_bab_ctx* c, std::vector<Run*>& act) {
+// The rest of a cutting-plane round (bounding.c:138-236) for runs whose
+// ADMM call has finished: tolerance, heuristic, prune and give-up tests,
+// separation; sets done for runs that leave the loop.
+int end_calls(madam_bab_ctx* c, std::vector<Run*>& act) {
   madam_params& P = c->params;
   int rc;
-  for (Run* r : act) {
-    ++r->count;
-    r->oldf = r->f;
-  }
-  double t0 = now_s();
-  if ((rc = admm_round(c, act))) return rc;
-  c->seconds_admm += now_s() - t0;
+  double t0;
   std::vector<Run*> heur;
   for (Run* r : act) {
     r->tol *= P.scaleTol;
@@ -651,6 +702,36 @@ int cut_round(madam_bab_ctx* c, std::vector<Run*>& act) {
   return fetch_fracsol(act, c->N);
 }
 
+// Cutting-plane rounds of a set of runs.  A step starts an ADMM call for
+// the runs that need one, advances the calls in flight and finishes the
+// round of those that stopped.  Without lockstep it returns once an eighth
+// of the calls (at least one) have stopped, so a run whose call converges
+// early starts its next round without waiting for the slowest one (used
+// for a fixed batch of nodes, where stragglers would leave the GPU idle);
+// with lockstep every call runs to its end (the B&B pipeline, which keeps
+// the GPU full by refilling slots and pays less per step that way).
+// `pending` holds the runs between calls.
+struct Rounds {
+  Flight F;
+  std::vector<Run*> pending;  // need a new ADMM call
+};
+
+int step_rounds(madam_bab_ctx* c, Rounds& R, bool lockstep) {
+  int rc;
+  double t0 = now_s();
+  if ((rc = begin_calls(c, R.pending, R.F))) return rc;
+  R.pending.clear();
+  std::vector<Run*> fin;
+  const int k = (int)R.F.runs.size();
+  if ((rc = advance_calls(c, R.F, lockstep ? 0 : std::max(1, k / 8), fin))) return rc;
+  c->seconds_admm += now_s() - t0;
+  if ((rc = end_calls(c, fin))) return rc;
+  for (Run* r : fin)
+    if (!r->done) R.pending.push_back(r);
+  return MADAM_OK;
+}
+
+
 // The bound a finished run reports (bounding.c:238-250).
 void finish_run(madam_bab_ctx* c, Run* r) {
   if (r->done != 2) {
@@ -673,29 +754,28 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
     all.push_back(&runs[q]);
   }
   if ((rc = start_runs(c, all))) return rc;
-  std::vector<Run*> act;
+  Rounds R;
   for (Run* r : all)
-    if (!r->done) act.push_back(r);
+    if (!r->done) R.pending.push_back(r);
   const double t_start = now_s();
-  int round = 0;
-  while (!act.empty()) {
-    ++round;
-    if ((rc = cut_round(c, act))) return rc;
-    std::vector<Run*> next;
-    for (Run* r : act)
-      if (!r->done) next.push_back(r);
+  int step = 0;
+  while (!R.pending.empty() || !R.F.runs.empty()) {
+    ++step;
+    if ((rc = step_rounds(c, R, false))) return rc;
     if (c->verbose > 0) {
-      int nbmax = 0;
+      int nbmax = 0, rounds = 0, left = 0;
       size_t mmax = 0;
-      for (Run* r : act) {
+      for (Run* r : all) {
         nbmax = std::max(nbmax, r->nbit);
+        rounds = std::max(rounds, r->count);
         mmax = std::max(mmax, r->tri.size() + r->pent.size() + r->hept.size());
+        left += r->done ? 0 : 1;
       }
-      fprintf(stderr, "sdpbound: round %d, %zu of %d nodes active, ADMM iterations <= %d, cuts <= %zu, %.2f s\n",
-              round, next.size(), k, nbmax, mmax, now_s() - t_start);
+      fprintf(stderr, "sdpbound: step %d, %d of %d nodes active (%zu ADMM calls in flight), rounds <= %d, "
+              "ADMM iterations <= %d, cuts <= %zu, %.2f s\n",
+              step, left, k, R.F.runs.size(), rounds, nbmax, mmax, now_s() - t_start);
       fflush(stderr);
     }
-    act.swap(next);
   }
   for (Run* r : all) {
     finish_run(c, r);
@@ -832,10 +912,11 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
     h.push(child_of(c, root, ic, 1));
   }
   delete root;
-  // Node pipeline: up to `batch` nodes in flight, one cutting-plane round of
-  // all of them per step; a node that finishes is branched at once and its
-  // slot refilled from the heap, so the batch stays full while the heap
-  // has nodes (in place of bounding `batch` nodes to the last straggler).
+  // Node pipeline: up to `batch` nodes in flight, each running its own
+  // cutting-plane rounds (step_rounds); a node that finishes is branched at
+  // once and its slot refilled from the heap, so the batch stays full while
+  // the heap has nodes (in place of bounding `batch` nodes to the last
+  // straggler).
   std::vector<std::unique_ptr<Run>> pool;
   std::vector<int> free_slots;
   for (int q = batch - 1; q >= 0; --q) free_slots.push_back(q);
@@ -851,6 +932,7 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
     for (auto* x : h.d) delete x;
     return code;
   };
+  Rounds R;
   for (;;) {
     std::vector<Run*> fresh;
     while ((int)pool.size() < batch && !h.d.empty() && (max_nodes <= 0 || started < max_nodes)) {
@@ -866,11 +948,12 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
       ++started;
     }
     if (pool.empty()) break;
-    if (!fresh.empty() && (rc = start_runs(c, fresh))) return fail(rc);
-    std::vector<Run*> act;
-    for (auto& r : pool)
-      if (!r->done) act.push_back(r.get());
-    if (!act.empty() && (rc = cut_round(c, act))) return fail(rc);
+    if (!fresh.empty()) {
+      if ((rc = start_runs(c, fresh))) return fail(rc);
+      for (Run* r : fresh)
+        if (!r->done) R.pending.push_back(r);
+    }
+    if ((!R.pending.empty() || !R.F.runs.empty()) && (rc = step_rounds(c, R, true))) return fail(rc);
     // finished nodes: bound, branch, free the slot
     for (size_t q = 0; q < pool.size();) {
       Run* r = pool[q].get();

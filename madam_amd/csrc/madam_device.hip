@@ -73,6 +73,18 @@ __device__ void spmv_bbt(const NodeView& nd, const double* __restrict__ p, doubl
   __syncthreads();
 }
 
+// Which cut-solve kernel serves a node: 0 k_cg1<2> (m <= 2 WG), 1 k_cg1<4>
+// (m <= 4 WG), 2 k_cg_lds, 3 k_cg (the LDS vectors do not fit).  Decided
+// per node, so that one large node does not move a whole batch to a
+// slower kernel; every cut-solve launch skips the nodes of other classes.
+__host__ __device__ inline int cg_class(int m, int nact, size_t lds_limit) {
+  const size_t base = ((size_t)m + nact + 64) * sizeof(double);
+  if (base > lds_limit) return 3;
+  if (m <= 2 * WG) return 0;
+  if (m <= 4 * WG) return 1;
+  return 2;
+}
+
 // Stopping tolerance of a cut solve: rtol, or with adapt > 0 after the
 // first iteration of an ADMM call max(rtol, min(1e-3, adapt * the previous
 // iteration's ADMM error)) -- an inexact solve while ADMM is far from
@@ -86,10 +98,11 @@ __device__ __forceinline__ double cg_tol(const NodeView& nd, double rtol, double
 // preconditioner, starting from the current t (the previous iterate), to a
 // relative residual of rtol.  Replaces the CHOLMOD factor/solve of
 // admm.c:486-551.
-__global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double adapt, int maxit) {
+__global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double adapt, int maxit,
+                                           size_t lds_limit) {
   __shared__ double red[4 * 16];
   NodeView& nd = nodes[blockIdx.x];
-  if (nd.done) return;
+  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != 3) return;
   const int m = nd.m;
   if (m == 0) return;
   double* __restrict__ x = nd.t;
@@ -194,10 +207,8 @@ __device__ inline void bbt_lds(const NodeView& nd, const Idx* __restrict__ aent,
 }
 
 template <class Idx>
-__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, double adapt, int maxit) {
+__device__ void cg_lds_body(NodeView& nd, double rtol, double adapt, int maxit) {
   extern __shared__ double smem[];
-  NodeView& nd = nodes[blockIdx.x];
-  if (nd.done) return;
   // 16-bit index streams halve the bytes each product reads (the launcher
   // picks them only when every node has m, nact < 2^15)
   const Idx* aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
@@ -279,6 +290,16 @@ __global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, dou
   }
 }
 
+__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, double adapt, int maxit,
+                                               size_t lds_limit) {
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != 2) return;
+  if (nd.aent16)
+    cg_lds_body<short>(nd, rtol, adapt, maxit);
+  else
+    cg_lds_body<int>(nd, rtol, adapt, maxit);
+}
+
 // sum of +-v[idx] over the signed entries ent[e0..e1) in entry order,
 // four loads in flight at a time (the sum itself stays sequential)
 template <class Idx>
@@ -319,11 +340,9 @@ __device__ __forceinline__ double signed_gather(const Idx* __restrict__ ent,
 //   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = D r,
 //   w = A u, gamma' = (r,u), delta = (w,u), beta = gamma' / gamma,
 //   alpha = gamma' / (delta - beta gamma' / alpha)
-template <int CPT, class Idx, bool STAGE>
-__global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double adapt, int maxit) {
+template <int CPT, class Idx>
+__device__ void cg1_body(NodeView& nd, double rtol, double adapt, int maxit, size_t lds_alloc) {
   extern __shared__ double smem[];
-  NodeView& nd = nodes[blockIdx.x];
-  if (nd.done) return;
   const Idx* aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
   const Idx* cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
   if (!aent) return;
@@ -333,7 +352,11 @@ __global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double
   double* w = smem + m;
   double* red = w + na;
   const int* aptr = nd.aptr;
-  if (STAGE) {
+  // the index streams are staged in LDS when this node's fit the allocation
+  const bool stage = ((size_t)m + na + 64) * sizeof(double) + ((size_t)na + 1) * sizeof(int) +
+                         2 * (size_t)nd.ne * sizeof(Idx) <=
+                     lds_alloc;
+  if (stage) {
     const int ne = nd.ne;
     int* sapt = (int*)(red + 64);
     Idx* sae = (Idx*)(sapt + na + 1);
@@ -499,6 +522,17 @@ __global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double
     nd.cg_iters = it;
     nd.cg_total += it;
   }
+}
+
+template <int CPT>
+__global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double adapt, int maxit,
+                                            size_t lds_limit, size_t lds_alloc) {
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != (CPT == 2 ? 0 : 1)) return;
+  if (nd.aent16)
+    cg1_body<CPT, short>(nd, rtol, adapt, maxit, lds_alloc);
+  else
+    cg1_body<CPT, int>(nd, rtol, adapt, maxit, lds_alloc);
 }
 
 // Cut duals (admm.c:557-560, 589-591) and the matrix to project
@@ -983,38 +1017,27 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   k_admm_prep<<<grid, WGS, 0, st>>>(nodes);
 }
 
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, double adapt,
-               int maxit, size_t lds_limit, hipStream_t st) {
-  const size_t lds = ((size_t)mmax + namax + 64) * sizeof(double);
-  const bool i16 = mmax < 32768 && namax < 32768;
-  // index streams staged in LDS as well: aptr (ints) and aent, cact
-  const size_t lds_st = lds + ((size_t)namax + 1) * sizeof(int) +
-                        2 * (size_t)nemax * (i16 ? sizeof(short) : sizeof(int));
-  const bool st_ok = lds_st <= lds_limit;
-#define MADAM_CG1(CPT)                                                              \
-  do {                                                                              \
-    if (i16 && st_ok)                                                               \
-      k_cg1<CPT, short, true><<<nb, WG, lds_st, st>>>(nodes, rtol, adapt, maxit);          \
-    else if (i16)                                                                   \
-      k_cg1<CPT, short, false><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);            \
-    else if (st_ok)                                                                 \
-      k_cg1<CPT, int, true><<<nb, WG, lds_st, st>>>(nodes, rtol, adapt, maxit);            \
-    else                                                                            \
-      k_cg1<CPT, int, false><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);              \
-  } while (0)
-  if (lds <= lds_limit && mmax <= 2 * WG) {
-    MADAM_CG1(2);
-  } else if (lds <= lds_limit && mmax <= 4 * WG) {
-    MADAM_CG1(4);
-  } else if (lds <= lds_limit) {
-    if (i16)
-      k_cg_lds<short><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);
-    else
-      k_cg_lds<int><<<nb, WG, lds, st>>>(nodes, rtol, adapt, maxit);
-  } else {
-    k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit);
+void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
+               size_t lds_limit, hipStream_t st) {
+  // per class: is any node in it, and the LDS its largest node asks for
+  // (vectors, plus the staged index streams when those fit)
+  size_t need[4] = {0, 0, 0, 0};
+  bool any[4] = {false, false, false, false};
+  for (int b = 0; b < nb; ++b) {
+    const NodeView& v = hv[b];
+    if (v.done || v.m == 0) continue;
+    const int cls = cg_class(v.m, v.nact, lds_limit);
+    any[cls] = true;
+    const size_t base = ((size_t)v.m + v.nact + 64) * sizeof(double);
+    const size_t staged = base + ((size_t)v.nact + 1) * sizeof(int) +
+                          2 * (size_t)v.ne * (v.aent16 ? sizeof(short) : sizeof(int));
+    const size_t want = cls <= 1 && staged <= lds_limit ? staged : base;
+    if (want > need[cls]) need[cls] = want;
   }
-#undef MADAM_CG1
+  if (any[0]) k_cg1<2><<<nb, WG, need[0], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[0]);
+  if (any[1]) k_cg1<4><<<nb, WG, need[1], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[1]);
+  if (any[2]) k_cg_lds<<<nb, WG, need[2], st>>>(nodes, rtol, adapt, maxit, lds_limit);
+  if (any[3]) k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit, lds_limit);
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

@@ -359,6 +359,7 @@ struct Engine {
   int cu_count = 256;
   std::vector<Slot> slots;
   DBuf<NodeView> views;
+  DBuf<NodeView> fviews;  // nodes whose ADMM call is being finished
   DBuf<double> tmpd;
   DBuf<double> postpart;  // k_admm_post partial sums: nodes x 64 workgroups x 8
   std::vector<NodeView> hviews;
@@ -521,6 +522,8 @@ inline int sync_views_down(int k) {
 // the k nodes whose views are in hv (device buffers already filled: L, X,
 // Q, y, cut structure, s = t = cut duals, u = 0).  hv is updated in place.
 int admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
+int admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax, int min_done);
+int admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
 
 // Triangle separation on the given slots (X current on the device).
 int tri_separate_slots(const std::vector<int>& slots, const int* K, double thresh,

@@ -33,6 +33,17 @@ Engine* g_eng = nullptr;
 using namespace mh;
 
 int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax) {
+  int rc = admm_advance(hv, o, nmax, mmax, 0);
+  if (rc) return rc;
+  return admm_finish(hv, o, nmax, mmax);
+}
+
+// Run the ADMM iterations of the views in hv (fresh calls or calls left
+// running by an earlier return) in chunks of check_every iterations until
+// every node has stopped, or, with min_done > 0, until at least min_done
+// of them have.  hv is brought up to date from the device after each chunk.
+int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax,
+                     int min_done) {
   Engine& E = *g_eng;
   const hipStream_t st = E.stream;
   const int k = (int)hv.size();
@@ -41,14 +52,14 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
   if ((rc = E.postpart.ensure((size_t)k * 64 * 8))) return rc;
   if ((rc = up(E.views.p, hv.data(), k, st))) return rc;
   const int chunk = std::max(1, o.check_every);
-  int itmax_all = 0, namax = 1, nemax = 1;
+  int itmax_all = 0, done0 = 0;
   for (int b = 0; b < k; ++b) {
-    itmax_all = std::max(itmax_all, hv[b].itermax);
-    namax = std::max(namax, hv[b].nact);
-    nemax = std::max(nemax, hv[b].ne);
+    itmax_all = std::max(itmax_all, hv[b].itermax - hv[b].nbit);
+    done0 += hv[b].done ? 1 : 0;
   }
   long launched = 0;
-  int active = k;
+  int active = k - done0;
+  if (active == 0) return MADAM_OK;
   for (;;) {
     if (launched > (long)itmax_all + 2 * chunk + 8) {
       set_err("ADMM stop flags never set after %ld iterations", launched);
@@ -72,7 +83,7 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
-            madam::launch_cg(E.views.p, k, mmax, namax, nemax, o.cg_rtol, o.cg_adapt, o.cg_maxit,
+            madam::launch_cg(E.views.p, hv.data(), k, o.cg_rtol, o.cg_adapt, o.cg_maxit,
                              E.lds_limit, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
@@ -95,25 +106,38 @@ int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int 
     E.timer.flush();
     active = 0;
     for (int b = 0; b < k; ++b) active += hv[b].done ? 0 : 1;
-    if (active == 0) break;
+    if (active == 0 || (min_done > 0 && k - done0 - active >= min_done)) break;
   }
+  return MADAM_OK;
+}
+
+// End of the ADMM calls in hv (all stopped): the rounding heuristic's
+// factor and the safe bound (admm.c:718-763), on their own view array.
+int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  const int k = (int)hv.size();
+  if (k == 0) return MADAM_OK;
+  int rc;
+  if ((rc = E.fviews.ensure(k))) return rc;
+  if ((rc = up(E.fviews.p, hv.data(), k, st))) return rc;
   if (o.projection == 1) {
     // the rounding heuristic's factor Z: pivoted Cholesky of X
-    TIMED(PH_ZFAC, st, madam::launch_zfac_chol(E.views.p, k, st));
+    TIMED(PH_ZFAC, st, madam::launch_zfac_chol(E.fviews.p, k, st));
   }
   TIMED(PH_SAFE, st, {
-    madam::launch_build_w(E.views.p, k, nmax, mmax, 1, st);
+    madam::launch_build_w(E.fviews.p, k, nmax, mmax, 1, st);
     // chosen form, nodes too large for the LDS-resident Jacobi: Lanczos +
     // Cholesky certificate; else all eigenvalues by Jacobi, as dsyevr in
     // the reference
     if (o.projection == 1 && madam::jacobi_lds_bytes(nmax, false) > E.lds_limit)
-      madam::launch_safe_lc(E.views.p, k, nmax, st);
+      madam::launch_safe_lc(E.fviews.p, k, nmax, st);
     else
-      madam::launch_jacobi(E.views.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
-    madam::launch_safe_bound(E.views.p, k, st);
+      madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
+    madam::launch_safe_bound(E.fviews.p, k, st);
   });
   HIP_TRY(hipGetLastError());
-  if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
+  if ((rc = down(hv.data(), E.fviews.p, k, st))) return rc;
   HIP_TRY(hipStreamSynchronize(st));
   E.timer.flush();
   return MADAM_OK;
@@ -187,6 +211,7 @@ void madam_close(void) {
   g_eng->views.release();
   g_eng->tmpd.release();
   g_eng->postpart.release();
+  g_eng->fviews.release();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   delete g_eng;

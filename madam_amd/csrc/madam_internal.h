@@ -69,8 +69,9 @@ struct NodeView {
 // Launch helpers implemented in madam_device.hip.  All take the device array
 // of nodes and the batch size, and run on `stream`.
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
-void launch_cg(NodeView* nodes, int nb, int mmax, int namax, int nemax, double rtol, double adapt,
-               int maxit, size_t lds_limit, hipStream_t stream);
+// hv: host copies of the views (per-node choice of the cut-solve kernel)
+void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
+               size_t lds_limit, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
                    int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
