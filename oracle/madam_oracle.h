@@ -2,9 +2,9 @@
  * madam_oracle.h -- CPU FP64 restatement of MADAM's SDP bounding path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (madam_amd/, include/)
- * links, loads or calls this code.  It is used by tests/ as the checker, by
- * __graft_entry__.smoke() as the checker and by bench.py's cpu_baseline leg
- * as the timed CPU path.
+ * links, loads or calls this code.  tests/ and __graft_entry__.smoke()
+ * compare the GPU results with it; bench.py's cpu_baseline leg times it as
+ * the CPU path.
  *
  * Every function restates one reference function (file:line cited at its
  * definition) in this project's own code.  LAPACK (dsyevr, dpotrf, dpotrs,

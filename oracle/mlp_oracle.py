@@ -1,5 +1,5 @@
 """CPU FP64 restatement of the learned warm start's network (test
-infrastructure only: the checker of madam_amd/csrc/madam_mlp.hip).
+infrastructure only: the values madam_amd/csrc/madam_mlp.hip is compared with).
 
 Same architecture and conventions as the HIP kernels: weights rounded to
 BF16 (nearest, ties to even) before use, biases unrounded, tanh at every

@@ -1,11 +1,12 @@
 """ctypes bindings for the CPU FP64 oracle (oracle/madam_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's cpu_baseline leg.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ and __graft_entry__.smoke() (the
+values the GPU results are compared with) and bench.py's cpu_baseline leg.
+The product never imports it.
 
 Also binds oracle/_ref/libmadam_ref.so (the reference's own operators.c,
 cutting_planes.c, qap_simulated_annealing.c, ipm_mc_pk.c, heap.c,
-allocate_free.c compiled from /root/reference by oracle/Makefile) when that
+allocate_free.c compiled from the reference's sources by oracle/Makefile) when that
 library exists; it is used only to pin the oracle.
 """
 from __future__ import annotations

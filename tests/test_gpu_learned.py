@@ -1,4 +1,4 @@
-"""Learned warm start on the GPU (madam_mlp.hip) against its checker.
+"""Learned warm start on the GPU (madam_mlp.hip) against its FP64 oracle.
 
   forward          within 2e-5 absolute of the FP64 oracle (same BF16 weights;
                    the kernels sum in FP32)

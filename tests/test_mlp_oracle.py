@@ -1,4 +1,4 @@
-"""CPU pins of the learned warm start's checker (oracle/mlp_oracle.py):
+"""CPU pins of the learned warm start's FP64 oracle (oracle/mlp_oracle.py):
   * BF16 rounding equals torch's float32 -> bfloat16 conversion (round to
     nearest, ties to even), an independent implementation;
   * the hand-written backward pass equals central finite differences of

@@ -15,7 +15,7 @@ Pins established here (bit-exact unless a tolerance is written):
   qap simulated annealing         qap_simulated_annealing.c:12-206
   pentagonal / heptagonal lists   cutting_planes.c:224-605
   ipm_mc_pk                       ipm_mc_pk.c:28-347   (1e-10 relative)
-Skipped when oracle/_ref is absent (it is built only where /root/reference
+Skipped when oracle/_ref is absent (it is built only where the reference tree
 exists).
 """
 import ctypes as C
