@@ -328,7 +328,8 @@ def wl_ns_batch(args):
         kk = k if form == "chosen" else min(k, 2)
         probs = [dict(L=L, X=np.eye(L.shape[0]), y=np.diag(L).copy(), Q=np.zeros_like(L),
                       sigma=1.0, tol=1e-14, itermax=64) for _ in range(kk)]
-        opts = dict(projection=1, cg_rtol=1e-7) if form == "chosen" else dict(projection=0)
+        opts = (dict(projection=1, cg_rtol=1e-7, ns_layout=args.ns_layout) if form == "chosen"
+                else dict(projection=0))
         res = G.ADMM_solver_batch(probs, **opts)
         it = sum(r.nbit for r in res)
         bs = np.array([r.f for r in res])
