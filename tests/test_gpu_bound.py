@@ -130,3 +130,50 @@ def test_root_bound_matches_reference_record(projection, cg_rtol):
           "rel", (b - rec["root_bound"]) / rec["root_bound"])
     assert b >= rec["optimum"]
     assert abs(b - rec["root_bound"]) <= 1e-4 * rec["root_bound"]
+
+
+def brute_force_fixed(SPL, xfixed, solX):
+    """Maximum cut value over the assignments that respect the fixings
+    (vertex N-1 on side 0, as the objective matrix assumes)."""
+    N = SPL.shape[0]
+    A = SPL[:N - 1, :N - 1]
+    free = [i for i in range(N - 1) if not xfixed[i]]
+    X = np.array(list(itertools.product([0, 1], repeat=len(free))), dtype=float)
+    full = np.zeros((X.shape[0], N - 1))
+    full[:, free] = X
+    for i in range(N - 1):
+        if xfixed[i]:
+            full[:, i] = solX[i]
+    return np.einsum("ij,jk,ik->i", full, A, full).max()
+
+
+@pytest.mark.parametrize("projection,cg_rtol,cg_adapt", [(0, 1e-12, 0.0), (1, 1e-7, 0.01)])
+def test_sdpbound_batch_of_fixed_nodes_is_valid(projection, cg_rtol, cg_adapt):
+    """Six nodes with different fixings bounded in one batch (each node on
+    its own cutting-plane schedule): every bound is at least the node's
+    brute-force optimum, and close to the same node bounded alone (they
+    differ only through the random streams of separation and rounding,
+    which are handed out in call order)."""
+    n = 18
+    SPL = I.objective_matrix(*I.gnp(n, 0.5, [-3, -2, -1, 1, 2, 3], 21))
+    rng = np.random.default_rng(5)
+    specs = []
+    for q in range(6):
+        xf = np.zeros(n - 1, np.int32)
+        xs = np.zeros(n - 1, np.int32)
+        idx = rng.choice(n - 1, size=q + 1, replace=False)
+        xf[idx] = 1
+        xs[idx] = rng.integers(0, 2, size=idx.size)
+        specs.append((xf, xs))
+    kw = dict(projection=projection, cg_rtol=cg_rtol, cg_adapt=cg_adapt, own_incumbent=1)
+    params = dict(use_diff=0)  # run the cutting-plane loop on every node
+    B = G.Bounder(SPL, G.default_params(**params), **kw)
+    nodes = [B.node(xf, xs) for xf, xs in specs]
+    bounds = B.sdpbound(nodes)
+    for q, (xf, xs) in enumerate(specs):
+        best = brute_force_fixed(SPL, xf, xs)
+        S = G.Bounder(SPL, G.default_params(**params), **kw)
+        alone = S.sdpbound([S.node(xf, xs)])[0]
+        print(q, "batch", bounds[q], "alone", alone, "brute force", best)
+        assert bounds[q] >= best - 1e-6
+        assert abs(bounds[q] - alone) <= 5e-3 * max(1.0, abs(alone))

@@ -50,7 +50,9 @@ def test_triangle_separation_at_admm_point():
         assert sorted(lst_g, key=key) == sorted(lst_o, key=key)
 
 
-@pytest.mark.parametrize("n,k,seed", [(15, 5, 11), (60, 5, 12), (60, 7, 13), (140, 7, 14)])
+# n = 15 and 60 read X from its copy in LDS, n = 140 and 200 from HBM
+# (an n x n FP64 copy does not fit the 160 KB of LDS of a workgroup)
+@pytest.mark.parametrize("n,k,seed", [(15, 5, 11), (60, 5, 12), (60, 7, 13), (140, 7, 14), (200, 5, 15)])
 def test_qap_sa_same_stream_as_oracle(n, k, seed):
     X = psd_X(n, seed)
     ntypes = 3 if k == 5 else 4
