@@ -35,12 +35,8 @@ __global__ __launch_bounds__(WGS) void k_admm_prep(NodeView* nodes) {
       nd.y[i] = L[dg] + Q[dg] + (X[dg] - 1.0) / sg;
     }
   for (int c = blockIdx.x * WGS + threadIdx.x; c < m; c += gridDim.x * WGS) {
-    double acc = 0.0;
-    for (int e = nd.cptr[c]; e < nd.cptr[c + 1]; ++e) {
-      const int en = nd.cent[e], lo = ent_idx(en);
-      const double v = L[lo] + Q[lo] + X[lo] / sg;
-      acc = en >= 0 ? acc + v : acc - v;
-    }
+    const double acc = signed_sum<false>(nd.cent, nd.cptr[c], nd.cptr[c + 1], 0.0,
+                                         [&](int lo) { return L[lo] + Q[lo] + X[lo] / sg; });
     double r = nd.s[c] + nd.cscale[c] * acc;
     nd.rhs[c] = r + (nd.u[c] - 1) / sg;
   }
@@ -565,11 +561,8 @@ __global__ __launch_bounds__(WGS) void k_build_w(NodeView* nodes, int use_s) {
     } else {
       const int a = nd.p2a[lo];
       if (a >= 0)
-        for (int e = nd.aptr[a]; e < nd.aptr[a + 1]; ++e) {
-          const int en = nd.aent[e], c = ent_idx(en);
-          const double temp = (0.5 * nd.cscale[c]) * vec[c];
-          w = en >= 0 ? w - temp : w + temp;
-        }
+        w = signed_sum<true>(nd.aent, nd.aptr[a], nd.aptr[a + 1], w,
+                             [&](int c) { return (0.5 * nd.cscale[c]) * vec[c]; });
     }
     nd.W[lo] = w;
     nd.W[j + i * n] = w;
@@ -870,12 +863,9 @@ __global__ __launch_bounds__(WGS) void k_post_part(NodeView* nodes, double* part
     }
   }
   for (int c = g * WGS + tid; c < m; c += stride) {
-    double acc = 0.0;
-    for (int e = nd.cptr[c]; e < nd.cptr[c + 1]; ++e) {
-      const int en = nd.cent[e];
-      const double x = sg * nd.S1[ent_idx(en)];
-      acc = en >= 0 ? acc + x : acc - x;
-    }
+    const double acc =
+        signed_sum<false>(nd.cent, nd.cptr[c], nd.cptr[c + 1], 0.0,
+                          [&](int lo) { return sg * nd.S1[lo]; });
     const double bx = -1.0 + nd.cscale[c] * acc;
     if (bx > 0) v[3] += bx * bx;
     const double ds = nd.s[c] - nd.t[c];
