@@ -159,9 +159,10 @@ int madam_ns_schedule(int cap, float *coef);
  * returns an upper bound lam >= lambda_max(A) within |E|_F + 1e-12 |A|_F. */
 int madam_lambda_max(int n, const double *A, double *lam);
 
-/* The same bound as the mixed-precision form computes it: a Lanczos
- * estimate certified by an FP64 Cholesky factorisation of mu I - A
- * (lam >= lambda_max(A), within ~1e-10 |A|_F). */
+/* The same bound as the mixed-precision form computes it: an estimate
+ * (Householder tridiagonalisation and Sturm multisection in LDS for n up to
+ * about 195, Lanczos beyond) certified by an FP64 Cholesky factorisation of
+ * mu I - A (lam >= lambda_max(A), within ~1e-10 |A|_F). */
 int madam_lambda_max_cert(int n, const double *A, double *lam);
 
 /* getViolated_TriangleInequalities (cutting_planes.c:48-127): the at most

@@ -192,6 +192,176 @@ __global__ __launch_bounds__(1024) void k_safe_lc(NodeView* nodes) {
 
 size_t safe_lc_lds_bytes(int nmax) { return 2 * (size_t)nmax * sizeof(double); }
 
+// The same certificate with a direct estimate, everything in LDS, for nodes
+// whose packed lower triangle fits there (n up to about 195).
+//
+// Estimate: A (packed lower triangle, order n) is reduced to tridiagonal
+// T = Q^T A Q by n - 2 Householder reflections H = I - tau v v^T (Golub and
+// Van Loan, Matrix Computations, Alg. 8.3.1: p = tau A v,
+// w = p - (tau/2)(p^T v) v, A <- A - v w^T - w v^T on the trailing block),
+// two barriers per reflection: after the product, after the update.  Every
+// wave computes the reflection's scalars itself (same data, same order,
+// same bits), so they need no barrier.  theta = largest eigenvalue of T by
+// multisection on the Sturm count: each of the 1024 threads counts the
+// eigenvalues above its own point of the bracket, five rounds shrink it by
+// 1025^5.  Certificate: Cholesky of mu I - A (A re-read from nd.W into the
+// same LDS triangle; chol_packed, one barrier per column), mu = theta +
+// delta, bound mu + 1.01 gamma_{n+1} trace(R^T R) + 2u (max|mu - a_jj| +
+// |mu|): the last term covers the rounding of the diagonal of mu I - A and
+// of the final sum.  Output as k_safe_lc.
+
+size_t safe_hh_lds_bytes(int n) {
+  return ((size_t)n * (n + 1) / 2 + 3 * (size_t)n) * sizeof(double);
+}
+
+__global__ __launch_bounds__(1024) void k_safe_hh(NodeView* nodes) {
+  extern __shared__ double sm[];
+  __shared__ double red[64];
+  __shared__ int flag[2];
+  __shared__ int sh_best;
+  NodeView& nd = nodes[blockIdx.x];
+  const int n = nd.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  if (n <= 0) return;
+  const double* __restrict__ Wm = nd.W;
+  double* P = sm;                              // packed lower triangle, order n
+  double* ta = P + (size_t)n * (n + 1) / 2;    // diagonal of T
+  double* tb = ta + n;                         // tb[i] = T(i, i-1)
+  double* pv = tb + n;                         // tau A v of the current reflection
+  double s2[1] = {0.0};
+  for (int f = tid; f < n * n; f += 1024) {
+    const int j = f / n, i = f - j * n;
+    const double a = Wm[f];
+    s2[0] += a * a;
+    if (i >= j) P[pk(i, j, n)] = a;
+  }
+  block_sums<1>(s2, red);  // (its barriers also publish P)
+  const double fro = sqrt(s2[0]);
+
+  // ---- tridiagonalisation
+  for (int j = 0; j + 2 < n; ++j) {
+    const int m = n - 1 - j;             // entries below the diagonal in column j
+    const int cj = pk(j + 1, j, n);      // P[cj + i] = A(j+1+i, j) = v_i (i >= 1)
+    const int bs = pk(j + 1, j + 1, n);  // trailing block, packed order m, from here
+    double sub = 0.0;
+    for (int i = 1 + lane; i < m; i += 64) sub += P[cj + i] * P[cj + i];
+    sub = wave_sum(sub);
+    const double x0 = P[cj];
+    double alpha = x0, v0 = 0.0, tau = 0.0;
+    if (sub > 0.0) {
+      const double nrm = sqrt(x0 * x0 + sub);
+      alpha = x0 >= 0.0 ? -nrm : nrm;
+      v0 = x0 - alpha;
+      tau = 2.0 / (v0 * v0 + sub);
+    }
+    if (tid == 0) {
+      ta[j] = P[pk(j, j, n)];
+      tb[j + 1] = alpha;
+    }
+    if (tau != 0.0) {  // the same in every thread
+      const int g = tid & 7;
+      for (int r = tid >> 3; r < m; r += 128) {  // 8 lanes per row
+        double acc = 0.0;
+        for (int k = g; k < m; k += 8) {
+          const double vk = k == 0 ? v0 : P[cj + k];
+          acc += P[bs + (r > k ? pk(r, k, m) : pk(k, r, m))] * vk;
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        if (g == 0) pv[r] = tau * acc;
+      }
+      __syncthreads();
+      double kk = 0.0;
+      for (int i = lane; i < m; i += 64) kk += (i == 0 ? v0 : P[cj + i]) * pv[i];
+      kk = 0.5 * tau * wave_sum(kk);
+      for (int c = wv; c < m; c += nw) {  // one wave per column of the block
+        const double vc = c == 0 ? v0 : P[cj + c];
+        const double wc = pv[c] - kk * vc;
+        double* col = P + bs + pk(c, c, m);
+        for (int r = c + lane; r < m; r += 64) {
+          const double vr = r == 0 ? v0 : P[cj + r];
+          const double wr = pv[r] - kk * vr;
+          col[r - c] -= vr * wc + wr * vc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    tb[0] = 0.0;
+    if (n >= 2) {
+      ta[n - 2] = P[pk(n - 2, n - 2, n)];
+      tb[n - 1] = P[pk(n - 1, n - 2, n)];
+    }
+    ta[n - 1] = P[pk(n - 1, n - 1, n)];
+  }
+  __syncthreads();
+
+  // ---- largest eigenvalue of T: bracket [max a_i, Gershgorin], multisection
+  double lo = ta[0], hi = ta[0];
+  for (int i = 0; i < n; ++i) {
+    const double rad = fabs(tb[i]) + (i + 1 < n ? fabs(tb[i + 1]) : 0.0);
+    lo = fmax(lo, ta[i]);
+    hi = fmax(hi, ta[i] + rad);
+  }
+  for (int round = 0; round < 5; ++round) {
+    const double x = lo + (hi - lo) * ((double)(tid + 1) / 1025.0);
+    int cnt = 0;  // eigenvalues of T above x: positive pivots of T - x I
+    double q = ta[0] - x;
+    if (q > 0.0) ++cnt;
+    for (int i = 1; i < n; ++i) {
+      const double den = fabs(q) < 1e-300 ? (q < 0.0 ? -1e-300 : 1e-300) : q;
+      q = (ta[i] - x) - tb[i] * tb[i] / den;
+      if (q > 0.0) ++cnt;
+    }
+    if (tid == 0) sh_best = -1;
+    __syncthreads();
+    if (cnt > 0) atomicMax(&sh_best, tid);
+    __syncthreads();
+    const int t = sh_best;
+    const double nlo = t >= 0 ? lo + (hi - lo) * ((double)(t + 1) / 1025.0) : lo;
+    const double nhi = t < 1023 ? lo + (hi - lo) * ((double)(t + 2) / 1025.0) : hi;
+    lo = nlo;
+    hi = nhi;
+    __syncthreads();  // sh_best is reset in the next round only after all have read it
+  }
+  const double theta = hi;
+
+  // ---- certificate
+  const double u = 1.1102230246251565e-16;  // 2^-53
+  const double gam = (n + 1) * u / (1.0 - (n + 1) * u);
+  double delta = 1e-11 * fro;
+  double cert = 1.01 * fro;  // lambda_max <= |A|_2 <= |A|_F
+  bool done = false;
+  while (!done && delta <= 2.0 * fro && fro > 0.0) {
+    const double mu = theta + delta;
+    __syncthreads();  // the previous attempt's readers of P are done
+    for (int f = tid; f < n * n; f += 1024) {
+      const int j = f / n, i = f - j * n;
+      if (i >= j) P[pk(i, j, n)] = i == j ? mu - Wm[f] : -Wm[f];
+    }
+    if (chol_packed(P, n, n, flag) == 0) {
+      double rr[1] = {0.0};
+      for (int f = tid; f < n * (n + 1) / 2; f += 1024) rr[0] += P[f] * P[f];
+      block_sums<1>(rr, red);
+      double dmx = 0.0;
+      for (int i = tid; i < n; i += 1024) dmx = fmax(dmx, fabs(mu - Wm[(size_t)i * (n + 1)]));
+      dmx = block_max(dmx, red);
+      const double c = mu + (1.01 * gam * rr[0] + 2.0 * u * (dmx + fabs(mu)));
+      cert = nextafter(c, INFINITY);
+      done = true;
+    } else {
+      delta *= 16.0;
+    }
+  }
+  for (int i = tid; i < n; i += 1024) nd.ev[i] = cert;
+  if (tid == 0) nd.off_last = 0.0;
+}
+
+void launch_safe_hh(NodeView* nodes, int nb, int nmax, hipStream_t st) {
+  k_safe_hh<<<nb, 1024, safe_hh_lds_bytes(nmax), st>>>(nodes);
+}
+
 void launch_safe_lc(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   k_safe_lc<<<nb, 1024, safe_lc_lds_bytes(nmax), st>>>(nodes);
 }

@@ -181,5 +181,60 @@ __device__ void wg_gemm(const double* __restrict__ A, int lda, const double* __r
     }
 }
 
+// Right-looking Cholesky of the packed lower triangle P (order n, leading
+// order n2) in place, one barrier per column: in step j wave w subtracts
+// column j's outer product from trailing columns k = j+1+w, j+1+w+nw, ...
+// (lanes down the column: contiguous LDS); the wave that owns column j+1
+// then finishes it (square root of its diagonal, column scaled), so it is
+// the pivot column of step j+1.  Returns 0, or c+1 for the first column c
+// with a non-positive pivot -- the same value in every thread.  flag: two
+// shared ints; a failure in step j is posted in flag[(j+1) & 1], which is
+// not written again before every thread has read it.
+__device__ inline int chol_packed(double* P, int n, int n2, int* flag) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  __syncthreads();  // readers of flag in a previous call are done
+  if (wv == 0) {
+    if (lane == 0) flag[0] = flag[1] = 0;
+    const double d = P[0];
+    if (d > 0.0) {
+      const double sd = sqrt(d);
+      for (int i = 1 + lane; i < n; i += 64) P[i] /= sd;  // pk(i, 0) = i
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) P[0] = sd;
+    } else if (lane == 0) {
+      flag[0] = 1;
+    }
+  }
+  __syncthreads();
+  if (flag[0]) return 1;
+  for (int j = 0; j + 1 < n; ++j) {
+    const int base = pk(j, j, n2);
+    for (int k = j + 1 + wv; k < n; k += nw) {
+      const double lk = P[base + (k - j)];
+      const int ck = pk(k, k, n2);
+      for (int i = k + lane; i < n; i += 64) P[ck + (i - k)] -= P[base + (i - j)] * lk;
+      if (k == j + 1) {
+        // the wave's own LDS writes are seen by its later reads (in order)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const double d = P[ck];
+        if (d > 0.0) {
+          const double sd = sqrt(d);
+          for (int i = k + 1 + lane; i < n; i += 64) P[ck + (i - k)] /= sd;
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (lane == 0) P[ck] = sd;
+        } else if (lane == 0) {
+          flag[(j + 1) & 1] = k + 1;
+        }
+      }
+    }
+    __syncthreads();
+    const int f = flag[(j + 1) & 1];
+    if (f) return f;
+  }
+  return 0;
+}
 
 }  // namespace madam

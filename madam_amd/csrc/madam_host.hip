@@ -127,10 +127,13 @@ int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, in
   }
   TIMED(PH_SAFE, st, {
     madam::launch_build_w(E.fviews.p, k, nmax, mmax, 1, st);
-    // chosen form, nodes too large for the LDS-resident Jacobi: Lanczos +
-    // Cholesky certificate; else all eigenvalues by Jacobi, as dsyevr in
-    // the reference
-    if (o.projection == 1 && madam::jacobi_lds_bytes(nmax, false) > E.lds_limit)
+    // chosen form: an estimate of lambda_max proved an upper bound by a
+    // Cholesky certificate -- the estimate by Householder tridiagonalisation
+    // in LDS, or by Lanczos for nodes too large for that; plain form: all
+    // eigenvalues by Jacobi, as dsyevr in the reference
+    if (o.projection == 1 && madam::safe_hh_lds_bytes(nmax) <= E.lds_limit)
+      madam::launch_safe_hh(E.fviews.p, k, nmax, st);
+    else if (o.projection == 1)
       madam::launch_safe_lc(E.fviews.p, k, nmax, st);
     else
       madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
@@ -412,7 +415,10 @@ int madam_lambda_max_cert(int n, const double* A, double* lam) {
   const hipStream_t st = g_eng->stream;
   if ((rc = up(sl.W.p, A, (size_t)n * n, st))) return rc;
   HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
-  madam::launch_safe_lc(g_eng->views.p, 1, n, st);
+  if (madam::safe_hh_lds_bytes(n) <= g_eng->lds_limit)
+    madam::launch_safe_hh(g_eng->views.p, 1, n, st);
+  else
+    madam::launch_safe_lc(g_eng->views.p, 1, n, st);
   madam::launch_safe_bound(g_eng->views.p, 1, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;
