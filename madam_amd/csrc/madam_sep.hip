@@ -362,22 +362,41 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
 }
 
 // sum_i X_ii + 2 sum_{j>i} |X_ji| in the reference's order
-// (qap_simulated_annealing.c:28-35), one lane per node.
-__global__ void k_sumx(const double* const* Xs, const int* ns, double* out, int nb) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const double* X = Xs[b];
+// (qap_simulated_annealing.c:28-35), one workgroup per node: the terms of a
+// run of columns are staged in LDS in that order (coalesced, all threads),
+// then one thread adds them in sequence -- the same bits as the plain loop.
+constexpr int SUMX_WG = 256, SUMX_BUF = 8192;
+
+__global__ __launch_bounds__(SUMX_WG) void k_sumx(const double* const* Xs, const int* ns,
+                                                 double* out) {
+#pragma clang fp contract(off)
+  __shared__ double buf[SUMX_BUF];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const double* __restrict__ X = Xs[b];
   const int n = ns[b];
   double acc = 0.0;
-  for (int i = 0; i < n; ++i) {
-    acc += X[i + (size_t)i * n];
-    for (int j = i + 1; j < n; ++j) acc += 2 * fabs(X[j + (size_t)i * n]);
+  for (int i0 = 0; i0 < n;) {
+    // columns [i0, i1): their n - i terms each fill at most SUMX_BUF slots
+    int i1 = i0, cnt = 0;
+    while (i1 < n && cnt + (n - i1) <= SUMX_BUF) cnt += n - i1++;
+    __syncthreads();  // the previous run's terms have been added
+    for (int f = i0 * n + tid; f < i1 * n; f += SUMX_WG) {
+      const int i = f / n, j = f - i * n;
+      if (j < i) continue;
+      // slot of (j, i): the terms of columns i0..i-1, then j - i
+      const int base = (i - i0) * n - ((i * (i - 1) - i0 * (i0 - 1)) >> 1);
+      buf[base + (j - i)] = j == i ? X[f] : 2 * fabs(X[f]);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int t = 0; t < cnt; ++t) acc += buf[t];
+    i0 = i1;
   }
-  out[b] = acc;
+  if (tid == 0) out[b] = acc;
 }
 
 void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipStream_t st) {
-  k_sumx<<<(nb + 63) / 64, 64, 0, st>>>(X, n, out, nb);
+  k_sumx<<<nb, SUMX_WG, 0, st>>>(X, n, out);
 }
 
 void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, int kk, SaOut* out,
