@@ -116,9 +116,10 @@ def _layout_graph(spec):
     return spec
 
 
-# orders 60 and 100: one-workgroup kernel with the iterates in LDS; 150: one
-# staged LDS buffer; 200: iterates in HBM
+# orders 60 and 100: one-workgroup kernel with the iterates in LDS; 120, 140
+# and 150: one staged LDS buffer; 200: iterates in HBM
 @pytest.mark.parametrize("graph,kgon", [("g05_60.0", False), ("w01_100.3", True),
+                                        ("gnp120", False), ("gnp140", False),
                                         ("gnp150", False), ("gnp200", False)])
 def test_tile_parallel_layout_is_bit_identical(graph, kgon):
     """The tile-parallel launch sequence computes the same tiles with the
@@ -132,6 +133,24 @@ def test_tile_parallel_layout_is_bit_identical(graph, kgon):
     assert a.nbit == b.nbit
     assert a.f == b.f
     assert np.array_equal(a.X, b.X)
+
+
+def test_batch_of_mixed_orders_is_bit_identical():
+    """One launch holding nodes of order 100, 130 and 150 (the storage mode
+    is the largest node's: one staged LDS buffer): each node's ADMM results
+    equal the tile-parallel layout's, node alone, bit for bit."""
+    cases = []
+    for g in ("gnp100", "gnp130", "gnp150"):
+        _, c = admm_case(graph_L(_layout_graph(g)))
+        c["tol"] = 1e-3
+        cases.append(c)
+    batch = G.ADMM_solver_batch(cases, projection=1, ns_layout=1)
+    for c, b in zip(cases, batch):
+        s = G.ADMM_solver(**c, projection=1, ns_layout=2)
+        print(c["L"].shape[0], "batch", b.f, b.nbit, "tiles alone", s.f, s.nbit)
+        assert s.nbit == b.nbit
+        assert s.f == b.f
+        assert np.array_equal(s.X, b.X)
 
 
 def test_heuristic_factor_reproduces_X():
