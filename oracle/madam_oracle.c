@@ -320,15 +320,20 @@ double mo_lambda_max(int n, const double *Ain) {
     double *zdummy = xcalloc(n, sizeof(double));
     int *iwork = xcalloc(10 * (size_t)n + 64, sizeof(int));
     int *isuppz = xcalloc(2 * (size_t)n + 2, sizeof(int));
+    /* dsyevr's W must hold n values whatever the range (it is also used
+     * as workspace); the selected eigenvalue is wv[0] */
+    double *wv = xcalloc(n, sizeof(double));
     int lwork = 26 * n + 64, liwork = 10 * n + 64, il = n, iu = n, m = 0, info;
-    double vl = 0, vu = 0, abstol = 1e-8, lam = 0.0;
+    double vl = 0, vu = 0, abstol = 1e-8;
     memcpy(A, Ain, nn * sizeof(double));
-    dsyevr_(&JN, &RI, &U, &n, A, &n, &vl, &vu, &il, &iu, &abstol, &m, &lam,
+    dsyevr_(&JN, &RI, &U, &n, A, &n, &vl, &vu, &il, &iu, &abstol, &m, wv,
             zdummy, &n, isuppz, work, &lwork, iwork, &liwork, &info);
     if (info != 0) {
         fprintf(stderr, "oracle: eigenvalue computation failed\n");
         exit(1);
     }
+    const double lam = wv[0];
+    free(wv);
     free(A);
     free(work);
     free(zdummy);

@@ -11,6 +11,11 @@ bab_functions.c need the rest of the program, MPI included):
   lambda_max of the safe bound (admm.c:727-747)
       the largest eigenvalue from numpy's eigvalsh, to dsyevr's absolute
       tolerance 1e-8 (the reference's ABSTOL), and not the second largest
+  ADMM (admm.c:49-771)                 on K_n with unit weights, whose basic
+      SDP bound is n^2/4 in closed form, the safe bound from the IPM start
+      without cuts is within 1e-6 of n^2/4 at tolerance 1e-7 and below it
+      by no more than n x 1e-8 (dsyevr's ABSTOL in the reference: its
+      lambda_max may fall short by that much)
   ADMM safe bound, SDPbound and B&B (admm.c, bounding.c, bab_functions.c)
       the root bound is >= and the B&B optimum == the brute-force maximum
       cut, on graphs of 10 to 14 vertices
@@ -112,3 +117,20 @@ def test_oracle_bound_and_bab_against_brute_force(n, p, weights, seed):
     assert root >= best - 1e-9
     opt = O.Solver(SPL, O.default_params()).bab()
     assert opt == best
+
+
+@pytest.mark.parametrize("n", [8, 11, 20])
+def test_admm_basic_sdp_of_complete_graph(n):
+    """K_n, unit weights: the Laplacian is n I - J, so <L, X> <= n^2 for
+    every feasible X (<J, X> >= 0) with equality at X = (n I - J) / (n - 1):
+    the basic SDP bound is n^2 / 4."""
+    ei, ej = zip(*itertools.combinations(range(1, n + 1), 2))
+    SPL = I.objective_matrix(n, np.array(ei), np.array(ej), np.ones(len(ei)))
+    p = O.default_params()
+    sol = O.Solver(SPL, p)
+    sol.load_subproblem()
+    sol.ipm()
+    f, _, nbit = sol.admm(p.sigma0, 1e-7)
+    print(n, "safe bound", f, "n^2/4", n * n / 4, "iterations", nbit)
+    assert f >= n * n / 4 - n * 1e-8
+    assert f <= n * n / 4 * (1 + 1e-6)
