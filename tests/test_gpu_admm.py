@@ -9,10 +9,13 @@ Bars (tolerances written here, FP64 throughout):
                                 1e-6 relative of the oracle
   batch                         each node equals its single-node run
 """
+import itertools
+
 import numpy as np
 import pytest
 
 import madam_amd as G
+from madam_amd import instances as I
 from oracle import oracle_py as O
 from tests.helpers import admm_case, graph_L, oracle_admm
 
@@ -123,3 +126,31 @@ def test_lambda_max_cert_is_an_upper_bound(n, seed, cluster):
     print(n, "cert", lam, "eigh", top, "excess / |A|_F", (lam - top) / fro)
     assert lam >= top - 1e-13 * fro
     assert lam - top <= 1e-8 * fro
+
+
+@pytest.mark.parametrize("n", [8, 20])
+def test_safe_bound_never_below_the_sdp_value(n):
+    """K_n with unit weights: the basic SDP bound is n^2/4 (the Laplacian is
+    n I - J, so <L, X> <= n^2 with equality at X = (n I - J) / (n - 1)); one
+    triangle cut, inactive at that X, leaves it unchanged.  From the IPM
+    start, run to 1e-7, both forms end within 1e-6 of n^2/4, and the chosen
+    form's certified bound is never below it (the reference's dsyevr value,
+    and so the oracle's, may fall short by up to n x 1e-8)."""
+    ei, ej = zip(*itertools.combinations(range(1, n + 1), 2))
+    SPL = I.objective_matrix(n, np.array(ei), np.array(ej), np.ones(len(ei)))
+    p = O.default_params()
+    sol = O.Solver(SPL, p)
+    sol.load_subproblem()
+    sol.ipm()
+    m = sol.s.n
+    case = dict(L=sol.mat("PPL").copy(order="F"), X=sol.mat("X").copy(order="F"),
+                Q=sol.mat("Q").copy(order="F"), y=sol.vec("y", m).copy(),
+                tri=[(1, 2, 1, 0, 0.0)], pent=[], hept=[], sigma=p.sigma0, tol=1e-7,
+                itermax=p.ADMM_itermax, scaleSigma=p.scaleSigma)
+    want = n * n / 4
+    for opts in (dict(projection=1, cg_rtol=1e-7), dict()):
+        got = G.ADMM_solver(**case, **opts)
+        print(n, opts, "safe bound", got.f, "n^2/4", want, "nbit", got.nbit)
+        assert abs(got.f - want) <= 1e-6 * want
+        if opts:
+            assert got.f >= want
