@@ -210,6 +210,8 @@ size_t safe_lc_lds_bytes(int nmax) { return 2 * (size_t)nmax * sizeof(double); }
 // |mu|): the last term covers the rounding of the diagonal of mu I - A and
 // of the final sum.  Output as k_safe_lc.
 
+// dynamic LDS of k_safe_hh; the kernel's static LDS (reductions, flags) adds
+// under 1 KB
 size_t safe_hh_lds_bytes(int n) {
   return ((size_t)n * (n + 1) / 2 + 3 * (size_t)n) * sizeof(double);
 }

@@ -131,7 +131,7 @@ int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, in
     // Cholesky certificate -- the estimate by Householder tridiagonalisation
     // in LDS, or by Lanczos for nodes too large for that; plain form: all
     // eigenvalues by Jacobi, as dsyevr in the reference
-    if (o.projection == 1 && madam::safe_hh_lds_bytes(nmax) <= E.lds_limit)
+    if (o.projection == 1 && madam::safe_hh_lds_bytes(nmax) + 1024 <= E.lds_limit)
       madam::launch_safe_hh(E.fviews.p, k, nmax, st);
     else if (o.projection == 1)
       madam::launch_safe_lc(E.fviews.p, k, nmax, st);
@@ -415,7 +415,7 @@ int madam_lambda_max_cert(int n, const double* A, double* lam) {
   const hipStream_t st = g_eng->stream;
   if ((rc = up(sl.W.p, A, (size_t)n * n, st))) return rc;
   HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
-  if (madam::safe_hh_lds_bytes(n) <= g_eng->lds_limit)
+  if (madam::safe_hh_lds_bytes(n) + 1024 <= g_eng->lds_limit)
     madam::launch_safe_hh(g_eng->views.p, 1, n, st);
   else
     madam::launch_safe_lc(g_eng->views.p, 1, n, st);
