@@ -1032,8 +1032,9 @@ void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double 
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {
   int work = nmax * nmax / 2 > mmax ? nmax * nmax / 2 : mmax;
-  // two lower entries per thread: each is a chain of dependent gathers
-  dim3 grid(chunks_for(work, 2 * WGS), nb);
+  // one lower entry per thread (up to 64 chunks a node): each is a chain of
+  // dependent gathers, so a lone node finishes sooner with more threads
+  dim3 grid(chunks_for(work, WGS), nb);
   k_build_w<<<grid, WGS, 0, st>>>(nodes, use_s);
 }
 
@@ -1063,7 +1064,7 @@ void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_s
 void launch_admm_post(NodeView* nodes, int nb, int nmax, int mmax, double* part,
                       hipStream_t st) {
   const int work = nmax * nmax / 2 > mmax ? nmax * nmax / 2 : mmax;
-  const int G = chunks_for(work, 2 * WGS);  // <= 64
+  const int G = chunks_for(work, WGS);  // <= 64
   k_post_part<<<dim3(G, nb), WGS, 0, st>>>(nodes, part);
   k_post_fin<<<nb, 64, 0, st>>>(nodes, part, G);
 }
