@@ -236,7 +236,8 @@ struct Slot {
   int size_ns(int n) {
     const size_t np = (size_t)((n + 15) & ~15);
     int rc;
-    if ((rc = ns32.ensure(3 * np * np)) || (rc = ip1.ensure((size_t)n * n))) return rc;
+    // the three FP32 iterates, then 16 doubles of the tile layout's norm sums
+    if ((rc = ns32.ensure(3 * np * np + 32)) || (rc = ip1.ensure((size_t)n * n))) return rc;
     return MADAM_OK;
   }
   DBuf<double> gwval, gwbest;            // GW per-hyperplane values, best value

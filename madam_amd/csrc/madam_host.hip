@@ -493,6 +493,23 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
       return rc;
   }
   HIP_TRY(hipStreamSynchronize(st));
+  // a boundary bin too full to rank (many equal or crowded values): the
+  // exact refinement for the flagged nodes, then their counts again
+  bool over = false;
+  for (int b = 0; b < k; ++b) over = over || ctl[(size_t)b * madam::CTL_N + madam::CTL_OVER] == 1;
+  if (over) {
+    TIMED(PH_TRISEP, st, madam::launch_tri_refine(E.sepviews.p, k, nmax, st));
+    HIP_TRY(hipGetLastError());
+    for (int b = 0; b < k; ++b)
+      if ((rc = down(ctl.data() + (size_t)b * madam::CTL_N, E.sep[b].ctl.p, madam::CTL_N, st)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int b = 0; b < k; ++b)
+      if (ctl[(size_t)b * madam::CTL_N + madam::CTL_OVER] == 1) {
+        set_err("triangle separation: boundary refinement did not finish");
+        return MADAM_E_HIP;
+      }
+  }
   std::vector<std::vector<uint32_t>> keys(k);
   std::vector<std::vector<double>> vals(k);
   for (int b = 0; b < k; ++b) {

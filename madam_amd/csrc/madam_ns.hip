@@ -375,49 +375,85 @@ void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_
 // than the chip has CUs (a single node, or the stragglers of a batch); the
 // arithmetic is the one of k_ns_project, tile for tile.
 
-// S_0 = W / |W|_F into buffer 0 of ns32 (zero padded), clear ns_fail.
-__global__ __launch_bounds__(1024) void k_ns_init(NodeView* nodes) {
-  __shared__ double red[64];
-  NodeView& nd = nodes[blockIdx.x];
+// S_0 = W / |W|_F into buffer 0 of ns32 (zero padded), clear ns_fail, in
+// two launches spread over many workgroups with the arithmetic of
+// k_ns_project: |W|_F^2 is summed as there by 1024 virtual threads (thread
+// v adds W[f]^2 for f = v, v + 1024, ... in order), each 64 of them
+// combined by wave_sum -- here one 64-thread workgroup per virtual wave,
+// its sum stored after the three iterates in ns32 -- and the 16 wave sums
+// added in order (block_sums).  Same bits as the one-workgroup layout.
+constexpr int NS_NORM_WAVES = WG / 64;
+
+__device__ __forceinline__ double* ns_norm_parts(const NodeView& nd) {
+  const size_t np = (size_t)((nd.n + 15) & ~15);
+  return (double*)(nd.ns32 + 3 * np * np);
+}
+
+// Workgroup-to-work maps of the tile-parallel kernels.  Batches of 8 or
+// more nodes: grid (8 x parts, ceil(nodes / 8)), workgroup (x, y) serves
+// node 8 y + x % 8 and part x / 8; workgroups go to the 8 XCDs round-robin
+// in dispatch order, so every part of a node runs on one XCD and the node's
+// operands are fetched into one L2.  Fewer nodes (`spread`): grid (parts,
+// nodes), so that a node's parts go round-robin to all 8 XCDs instead of
+// leaving seven of them idle.
+__device__ __forceinline__ int ns_node(int nb, int spread, int& part, int& nparts) {
+  if (spread) {
+    part = blockIdx.x;
+    nparts = gridDim.x;
+    return blockIdx.y;
+  }
+  part = blockIdx.x >> 3;
+  nparts = gridDim.x >> 3;
+  const int b = blockIdx.y * 8 + (blockIdx.x & 7);
+  return b < nb ? b : -1;
+}
+
+__global__ __launch_bounds__(64) void k_ns_norm(NodeView* nodes) {
+  NodeView& nd = nodes[blockIdx.y];
   if (nd.done) return;
-  const int n = nd.n, np = (n + 15) & ~15, tid = threadIdx.x;
-  const size_t npp = (size_t)np * np;
+  const int n = nd.n;
   const double* __restrict__ Wm = nd.W;
-  double fv[1] = {0.0};
-  for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
-  block_sums<1>(fv, red);
-  const double fro = sqrt(fv[0]);
+  double fv = 0.0;
+  for (int f = blockIdx.x * 64 + threadIdx.x; f < n * n; f += WG) fv += Wm[f] * Wm[f];
+  fv = wave_sum(fv);
+  if (threadIdx.x == 0) ns_norm_parts(nd)[blockIdx.x] = fv;
+}
+
+__global__ __launch_bounds__(256) void k_ns_fill(NodeView* nodes, int nb, int spread) {
+  int part, nparts;
+  const int b = ns_node(nb, spread, part, nparts);
+  if (b < 0) return;
+  NodeView& nd = nodes[b];
+  if (nd.done) return;
+  const int n = nd.n, np = (n + 15) & ~15;
+  const size_t npp = (size_t)np * np;
+  const double* parts = ns_norm_parts(nd);
+  double r = 0.0;
+  for (int i = 0; i < NS_NORM_WAVES; ++i) r += parts[i];
+  const double fro = sqrt(r);
   const float inv = fro > 0.0 ? (float)(1.0 / fro) : 0.f;
+  const double* __restrict__ Wm = nd.W;
   float* Sb = nd.ns32;
-  for (size_t f = tid; f < npp; f += WG) {
+  for (size_t f = (size_t)part * 256 + threadIdx.x; f < npp; f += (size_t)nparts * 256) {
     const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
     Sb[f] = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
   }
-  if (tid == 0) nd.ns_fail = 0;
+  if (part == 0 && threadIdx.x == 0) nd.ns_fail = 0;
 }
 
 // C = A (alpha I + beta B) on buffers ia, ib, ic of ns32: one workgroup of
 // four waves per tile, wave w accumulating K quarter w (split K: a quarter
 // of the latency of one wave doing the whole tile), the partial tiles summed
 // in LDS in the quarter order of gemm_tile_f32.
-// Grids of the per-tile kernels are (8 x tiles, ceil(nodes / 8)): workgroup
-// (x, y) serves node 8 y + x % 8 and tile x / 8.  Workgroups go to the 8
-// XCDs round-robin in dispatch order, so every tile of a node runs on the
-// same XCD and the node's operand panels are fetched into one L2 only.
-__device__ __forceinline__ int xcd_node(int nb) {
-  const int b = blockIdx.y * 8 + (blockIdx.x & 7);
-  return b < nb ? b : -1;
-}
-
-__global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int ia, int ib, int ic,
-                                                   float alpha, float beta) {
+__global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int spread, int ia,
+                                                   int ib, int ic, float alpha, float beta) {
   __shared__ f32x4 part[3][64];
-  const int b = xcd_node(nb);
+  int t, nt_unused;
+  const int b = ns_node(nb, spread, t, nt_unused);
   if (b < 0) return;
   NodeView& nd = nodes[b];
   if (nd.done) return;
   const int np = (nd.n + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x >> 3;
   if (t >= ntile) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i0, j0;
@@ -433,16 +469,16 @@ __global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int 
 }
 
 // FP32 iterate (buffer is) -> FP64 n x n in S2, with the guard.
-__global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int is) {
-  const int b = xcd_node(nb);
+__global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int spread, int is) {
+  int part, nparts;
+  const int b = ns_node(nb, spread, part, nparts);
   if (b < 0) return;
   NodeView& nd = nodes[b];
   if (nd.done) return;
   const int n = nd.n, np = (n + 15) & ~15;
   const float* S = nd.ns32 + (size_t)is * np * np;
   int bad = 0;
-  const int chunks = gridDim.x >> 3;
-  for (int f = (blockIdx.x >> 3) * 256 + threadIdx.x; f < n * n; f += chunks * 256) {
+  for (int f = part * 256 + threadIdx.x; f < n * n; f += nparts * 256) {
     const int j = f / n, i = f - j * n;
     const float v = S[i + (size_t)j * np];
     if (!(fabsf(v) <= 1.5f)) bad = 1;
@@ -453,14 +489,15 @@ __global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int is
 
 // The FP64 finish, one product per launch (split K as above): 0: S1 = S2^2;
 // 1: ip1 = S2 (1.5 I - 0.5 S1); 2: S1 = W (0.5 I + 0.5 ip1) = X_+.
-__global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int which) {
+__global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int spread,
+                                                   int which) {
   __shared__ f64x4 part[3][64];
-  const int b = xcd_node(nb);
+  int t, nt_unused;
+  const int b = ns_node(nb, spread, t, nt_unused);
   if (b < 0) return;
   NodeView& nd = nodes[b];
   if (nd.done || nd.ns_fail) return;
   const int n = nd.n, nt = (n + 15) >> 4, ntile = nt * (nt + 1) / 2;
-  const int t = blockIdx.x >> 3;
   if (t >= ntile) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i0, j0;
@@ -479,23 +516,27 @@ __global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int 
 void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   static const float coef[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
   const int np = (nmax + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
+  const int spread = nb < 8 ? 1 : 0;
   const int ny = (nb + 7) / 8;
-  const dim3 g32(8 * ntile, ny);
-  k_ns_init<<<nb, 1024, 0, st>>>(nodes);  // workgroup b on XCD b % 8 as well
+  auto grid = [&](int parts) { return spread ? dim3(parts, nb) : dim3(8 * parts, ny); };
+  const dim3 g32 = grid(ntile);
+  k_ns_norm<<<dim3(NS_NORM_WAVES, nb), 64, 0, st>>>(nodes);
+  const int nfill = std::min(64, (np * np + 255) / 256);
+  k_ns_fill<<<grid(nfill), 256, 0, st>>>(nodes, nb, spread);
   int sb = 0, tb = 1, ub = 2;
   for (int k = 0; k < MADAM_NS_STEPS; ++k) {
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, sb, sb, tb, 0.f, 1.f);               // T = S^2
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, tb, tb, ub, coef[k][1], coef[k][2]);  // U
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, sb, ub, tb, coef[k][0], 1.f);         // S'
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, sb, sb, tb, 0.f, 1.f);  // T = S^2
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, tb, tb, ub, coef[k][1],
+                                     coef[k][2]);                                // U
+    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, sb, ub, tb, coef[k][0], 1.f);  // S'
     const int x = sb;
     sb = tb;
     tb = x;
   }
   const int nconv = std::min(64, (nmax * nmax + 255) / 256);
-  k_ns_conv<<<dim3(8 * nconv, ny), 256, 0, st>>>(nodes, nb, sb);
+  k_ns_conv<<<grid(nconv), 256, 0, st>>>(nodes, nb, spread, sb);
   const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
-  const dim3 g64(8 * ntile64, ny);
-  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<g64, 256, 0, st>>>(nodes, nb, w);
+  for (int w = 0; w < 3; ++w) k_ns_gemm64<<<grid(ntile64), 256, 0, st>>>(nodes, nb, spread, w);
 }
 
 // ---------------------------------------------------------------------------

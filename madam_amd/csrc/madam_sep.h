@@ -8,7 +8,15 @@ namespace madam {
 
 constexpr int SEP_NBIN = 4096;   // histogram bins over [thresh, max]
 constexpr int SEP_BCAP = 16384;  // boundary-bin capacity per node
-enum { CTL_COUNT = 0, CTL_NOUT, CTL_NBOUND, CTL_BSTAR, CTL_NEED, CTL_NFINAL, CTL_N };
+constexpr int SEP_RLEVELS = 8;   // 12-bit digits of the 96-bit refinement key
+// Control words of one node's separation.  CTL_OVER: 1 when the boundary
+// bin held more than SEP_BCAP rows (the exact refinement then runs), 2 once
+// the refinement has resolved it.  CTL_RMODE/RL/RNEED/RD0..: refinement
+// state (mode, digits fixed, rows still needed, the fixed digits).
+enum {
+  CTL_COUNT = 0, CTL_NOUT, CTL_NBOUND, CTL_BSTAR, CTL_NEED, CTL_NFINAL, CTL_OVER, CTL_RMODE,
+  CTL_RL, CTL_RNEED, CTL_RD0, CTL_N = CTL_RD0 + SEP_RLEVELS
+};
 
 // Per-node triangle separation state (all device pointers).
 struct SepView {
@@ -26,6 +34,9 @@ struct SepView {
 };
 
 void launch_tri_separate(SepView* sv, int nb, int nmax, hipStream_t st);
+// the exact refinement for nodes whose boundary bin overflowed (CTL_OVER
+// = 1 after launch_tri_separate; a no-op for the others), then the list
+void launch_tri_refine(SepView* sv, int nb, int nmax, hipStream_t st);
 
 // One simulated-annealing run (qap_simulated_annealing.c).
 constexpr int SA_WG = 64;

@@ -163,15 +163,34 @@ __global__ __launch_bounds__(SEP_WG) void k_tri_scan(SepView* sv) {
 // One workgroup per node: finish the list.  Boundary rows are ranked by
 // (value desc, key asc) and the first `need` kept; the list is then written
 // in key (= enumeration) order with the row values recomputed from X.
+// A boundary bin with more rows than SEP_BCAP (many equal or crowded
+// values) is not ranked here: the node is flagged (CTL_OVER = 1), its
+// refinement state is set up, and the list is finished by the AFTER
+// instance once k_tri_refine_* have selected the boundary exactly.
+template <bool AFTER>
 __global__ __launch_bounds__(1024) void k_tri_finish(SepView* sv) {
   SepView& s = sv[blockIdx.x];
   const int tid = threadIdx.x, NT = blockDim.x;
+  if (AFTER && s.ctl[CTL_OVER] != 1) return;
   const uint32_t count = s.ctl[CTL_COUNT];
   uint32_t nout = s.ctl[CTL_NOUT];
   if (nout > (uint32_t)s.K) nout = s.K;
   if (count > (uint32_t)s.K) {
     uint32_t nb = s.ctl[CTL_NBOUND];
-    if (nb > SEP_BCAP) nb = SEP_BCAP;
+    if (!AFTER && nb > (uint32_t)SEP_BCAP) {
+      // the pass-1 histogram is spent: it becomes the refinement's
+      for (int b = tid; b < SEP_NBIN; b += NT) s.hist[b] = 0;
+      if (tid == 0) {
+        s.ctl[CTL_OVER] = 1;
+        s.ctl[CTL_RMODE] = 1;
+        s.ctl[CTL_RL] = 0;
+        s.ctl[CTL_RNEED] = s.ctl[CTL_NEED];
+        s.ctl[CTL_NBOUND] = 0;
+        s.ctl[CTL_NFINAL] = 0;
+      }
+      return;
+    }
+    if (nb > SEP_BCAP) nb = SEP_BCAP;  // not reached: the refinement collects at most SEP_BCAP
     const uint32_t need = s.ctl[CTL_NEED];
     for (uint32_t q = tid; q < nb; q += NT) {
       const double v = s.bval[q];
@@ -201,7 +220,134 @@ __global__ __launch_bounds__(1024) void k_tri_finish(SepView* sv) {
   }
   if (tid == 0) {
     s.ctl[CTL_NFINAL] = nout;
+    if (AFTER) s.ctl[CTL_OVER] = 2;
   }
+}
+
+// ---- exact selection inside an overflowing boundary bin
+//
+// The rows of the boundary bin are ordered by the 96-bit key
+// C = (ord(value) : 64 bits, ~key : 32 bits), so that C descending is
+// (value descending, enumeration order ascending), the order the list
+// keeps.  A most-significant-digit radix selection on 12-bit digits fixes
+// C's digits one level at a time: each level histograms the next digit of
+// the rows that match the digits fixed so far, the scan picks the digit d*
+// holding the `need`-th largest, the rows above d* join the list, and the
+// rows at d* are either refined further or -- once they number at most
+// SEP_BCAP (at the latest after all 8 digits, when one row is left) --
+// collected for k_tri_finish<true>, which ranks them as usual.
+// Refinement modes (CTL_RMODE): 1 histogram digit 0; 2 emit the rows above
+// the last fixed digit and histogram the next one; 3 emit and collect;
+// 4 done.
+
+// digit p (0 = most significant) of C
+__device__ __forceinline__ uint32_t ref_digit(unsigned long long o, uint32_t nk, int p) {
+  const int off = 84 - 12 * p;  // lowest bit of digit p in C
+  if (off >= 32) return (uint32_t)(o >> (off - 32)) & 0xFFFu;
+  if (off + 12 <= 32) return (nk >> off) & 0xFFFu;
+  return (uint32_t)((o << (32 - off)) | (unsigned long long)(nk >> off)) & 0xFFFu;
+}
+
+__global__ __launch_bounds__(SEP_WG) void k_tri_refine_pass(SepView* sv) {
+  __shared__ double xi[1024];
+  __shared__ uint32_t hist[SEP_NBIN];
+  SepView& s = sv[blockIdx.y];
+  const uint32_t mode = s.ctl[CTL_RMODE];
+  if (mode == 0 || mode >= 4) return;
+  const int n = s.n, i = blockIdx.x;
+  if (i >= n || i < 2) return;
+  const int L = (int)s.ctl[CTL_RL];
+  uint32_t rd[SEP_RLEVELS];
+#pragma unroll
+  for (int p = 0; p < SEP_RLEVELS; ++p) rd[p] = p < L ? s.ctl[CTL_RD0 + p] : 0u;
+  const double* __restrict__ X = s.X;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < i; j += SEP_WG) xi[j] = X[i + (size_t)j * n];
+  const bool counting = mode <= 2;
+  if (counting)
+    for (int b = tid; b < SEP_NBIN; b += SEP_WG) hist[b] = 0;
+  __syncthreads();
+  const double thr = s.thresh, lo = thr;
+  const double vmax = from_ord(s.maxbits[0]);
+  const double scale = (vmax > thr) ? (double)SEP_NBIN / (vmax - thr) : 0.0;
+  const int bstar = (int)s.ctl[CTL_BSTAR];
+  const int npairs = i * (i - 1) / 2;
+  for (int p = tid; p < npairs; p += SEP_WG) {
+    int j, k;
+    pair_decode(p, j, k);
+    const double a = xi[j], b = xi[k], c = X[k + (size_t)j * n];
+#pragma unroll
+    for (int type = 1; type <= 4; ++type) {
+      const double v = tri_val(type, a, b, c);
+      if (!(v > thr)) continue;
+      int bin = (int)((v - lo) * scale);
+      bin = bin < 0 ? 0 : (bin >= SEP_NBIN ? SEP_NBIN - 1 : bin);
+      if (bin != bstar) continue;
+      const uint32_t key = tri_key(type, i, j, k);
+      const unsigned long long o = ord_bits(v);
+      const uint32_t nk = ~key;
+      if (mode == 1) {
+        atomicAdd(&hist[ref_digit(o, nk, 0)], 1u);
+        continue;
+      }
+      bool match = true;
+      for (int q = 0; q + 1 < L; ++q) match = match && ref_digit(o, nk, q) == rd[q];
+      if (!match) continue;
+      const uint32_t d = ref_digit(o, nk, L - 1);
+      const uint32_t dstar = rd[L - 1];
+      if (d > dstar) {
+        const uint32_t slot = atomicAdd(&s.ctl[CTL_NOUT], 1u);
+        if (slot < (uint32_t)s.K) s.okey[slot] = key;
+      } else if (d == dstar) {
+        if (mode == 2) {
+          atomicAdd(&hist[ref_digit(o, nk, L)], 1u);
+        } else {
+          const uint32_t slot = atomicAdd(&s.ctl[CTL_NBOUND], 1u);
+          if (slot < (uint32_t)SEP_BCAP) {
+            s.bkey[slot] = key;
+            s.bval[slot] = v;
+          }
+        }
+      }
+    }
+  }
+  if (counting) {
+    __syncthreads();
+    for (int b = tid; b < SEP_NBIN; b += SEP_WG)
+      if (hist[b]) atomicAdd(&s.hist[b], hist[b]);
+  }
+}
+
+// One workgroup per node: fix the next digit from the histogram, or end
+// the refinement after the collecting pass.
+__global__ __launch_bounds__(SEP_WG) void k_tri_refine_scan(SepView* sv) {
+  SepView& s = sv[blockIdx.x];
+  const uint32_t mode = s.ctl[CTL_RMODE];
+  if (mode == 0 || mode >= 4) return;
+  if (mode == 3) {
+    if (threadIdx.x == 0) {
+      s.ctl[CTL_NEED] = s.ctl[CTL_RNEED];
+      s.ctl[CTL_RMODE] = 4;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t need = s.ctl[CTL_RNEED];
+    uint32_t above = 0;
+    int d = SEP_NBIN - 1;
+    for (; d > 0; --d) {
+      if (above + s.hist[d] >= need) break;
+      above += s.hist[d];
+    }
+    const uint32_t at = s.hist[d];
+    const uint32_t L = s.ctl[CTL_RL];
+    s.ctl[CTL_RD0 + L] = (uint32_t)d;
+    s.ctl[CTL_RL] = L + 1;
+    s.ctl[CTL_RNEED] = need - above;
+    s.ctl[CTL_RMODE] = (at <= (uint32_t)SEP_BCAP || L + 1 == (uint32_t)SEP_RLEVELS) ? 3u : 2u;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < SEP_NBIN; b += SEP_WG) s.hist[b] = 0;
 }
 
 void launch_tri_separate(SepView* sv, int nb, int nmax, hipStream_t st) {
@@ -210,7 +356,17 @@ void launch_tri_separate(SepView* sv, int nb, int nmax, hipStream_t st) {
   k_tri_pass<1><<<grid, SEP_WG, 0, st>>>(sv);
   k_tri_scan<<<nb, SEP_WG, 0, st>>>(sv);
   k_tri_pass<2><<<grid, SEP_WG, 0, st>>>(sv);
-  k_tri_finish<<<nb, 1024, 0, st>>>(sv);
+  k_tri_finish<false><<<nb, 1024, 0, st>>>(sv);
+}
+
+void launch_tri_refine(SepView* sv, int nb, int nmax, hipStream_t st) {
+  dim3 grid(nmax, nb);
+  // mode 1, then at most SEP_RLEVELS - 1 levels in mode 2, then mode 3
+  for (int l = 0; l <= SEP_RLEVELS; ++l) {
+    k_tri_refine_pass<<<grid, SEP_WG, 0, st>>>(sv);
+    k_tri_refine_scan<<<nb, SEP_WG, 0, st>>>(sv);
+  }
+  k_tri_finish<true><<<nb, 1024, 0, st>>>(sv);
 }
 
 // ------------------------------------------------------------------ SA

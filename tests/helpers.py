@@ -19,10 +19,13 @@ def graph_L(spec):
     return I.objective_matrix(nv, ei, ej, w)
 
 
-def admm_case(SPL, kgon=False, seed=7, xfixed=None, solX=None):
+def admm_case(SPL, kgon=False, seed=7, xfixed=None, solX=None, params=None, hept=True):
     """Oracle state after IPM + first separation: the inputs of the first
-    ADMM_solver call of SDPbound (bounding.c:70-128)."""
-    p = O.default_params()
+    ADMM_solver call of SDPbound (bounding.c:70-128).  kgon: pentagonal
+    (and, with hept, heptagonal) cuts separated at the IPM point as well,
+    so that the call runs on all cut families; params: overrides of the
+    params-file values."""
+    p = O.default_params(**(params or {}))
     sol = O.Solver(SPL, p)
     sol.load_subproblem(xfixed, solX)
     sol.ipm()
@@ -31,8 +34,9 @@ def admm_case(SPL, kgon=False, seed=7, xfixed=None, solX=None):
     if kgon:
         libc.srand(seed)
         O.lib().mo_pent_update(sol.st, sol.s.s, C.c_int(sol.s.ntri), C.byref(a), C.byref(r))
-        O.lib().mo_hept_update(sol.st, sol.s.s, C.c_int(sol.s.ntri + sol.s.npent), C.byref(a),
-                               C.byref(r))
+        if hept:
+            O.lib().mo_hept_update(sol.st, sol.s.s, C.c_int(sol.s.ntri + sol.s.npent),
+                                   C.byref(a), C.byref(r))
     s = sol.s
     tri = [(s.cuts[q].type, s.cuts[q].i, s.cuts[q].j, s.cuts[q].k, s.cuts[q].y) for q in range(s.ntri)]
     pent = [(s.pcuts[q].type, tuple(s.pcuts[q].perm), s.pcuts[q].y) for q in range(s.npent)]

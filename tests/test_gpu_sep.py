@@ -50,6 +50,42 @@ def test_triangle_separation_at_admm_point():
         assert sorted(lst_g, key=key) == sorted(lst_o, key=key)
 
 
+def _overflow_X(kind, n=60):
+    """X whose violated triangle rows crowd the top histogram bin: more than
+    the 16384 rows the boundary bin holds (then the exact refinement runs)."""
+    rng = np.random.default_rng(8)
+    if kind == "tied":          # all C(n,3) type-1 rows equal 0.5 exactly
+        X = np.full((n, n), -0.5)
+    elif kind == "crowded":     # distinct values within 1e-8 of each other
+        X = -0.5 + 1e-9 * rng.standard_normal((n, n))
+    else:                       # ties on a few levels plus a higher block
+        X = np.full((n, n), -0.5)
+        X[:6, :6] = -0.9
+    X = (X + X.T) / 2
+    np.fill_diagonal(X, 1.0)
+    return np.asfortranarray(X)
+
+
+# (block, 20000) is left out: there the K-th value is tied between rows
+# found before and after the list filled, where the reference's slot
+# replacement keeps later rows and this library the earlier ones
+@pytest.mark.parametrize("kind,K", [("tied", 600), ("tied", 20000), ("crowded", 600),
+                                    ("crowded", 20000), ("block", 600)])
+def test_triangle_separation_boundary_overflow(kind, K):
+    """A boundary bin with more than 16384 rows is selected exactly (value
+    descending, then enumeration order), the same set as the oracle's and
+    the same list on every call."""
+    X = _overflow_X(kind)
+    mx_g, lst_g = G.tri_separate(X, K, 1e-3)
+    mx_o, lst_o = O.tri_separate(X, K, 1e-3)
+    assert mx_g == mx_o
+    key = lambda c: (c[0], c[1], c[2], c[3])
+    assert len(lst_g) == len(lst_o) == K
+    assert sorted(lst_g, key=key) == sorted(lst_o, key=key)
+    for _ in range(2):
+        assert G.tri_separate(X, K, 1e-3)[1] == lst_g
+
+
 # n = 15 and 60 read X from its copy in LDS, n = 140 and 200 from HBM
 # (an n x n FP64 copy does not fit the 160 KB of LDS of a workgroup)
 @pytest.mark.parametrize("n,k,seed", [(15, 5, 11), (60, 5, 12), (60, 7, 13), (140, 7, 14), (200, 5, 15)])
