@@ -173,6 +173,21 @@ int madam_lambda_max_cert(int n, const double *A, double *lam);
 int madam_tri_separate(int n, const double *X, int K, double thresh, madam_tri *list, int *size,
                        double *maxval);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): madam_tri_separate on device buffers, allocating
+ * nothing.  X (n x n), list (K entries), size and maxval are DEVICE
+ * pointers; the work is ordered on `stream` (a hipStream_t; null is HIP's
+ * default stream) and the call returns without waiting for it.  Same list,
+ * in the same order, as madam_tri_separate.  Working memory is one fixed
+ * device array per GPU (1,013,760 bytes, held until the process ends),
+ * which limits 3 <= n <= MADAM_NMAX and 0 <= K <= 50000 (the reference's
+ * MaxTriIneqAdded); other sizes are rejected with MADAM_E_ARG.  Calls that
+ * overlap on one GPU share that array and race: serialise them.
+ * madam_open(device) must have been called for the current device. */
+int madam_tri_separate_noalloc(int n, const double *X, int K, double thresh, madam_tri *list,
+                               int *size, double *maxval, void *stream);
+
 /* qap_simulated_annealing (qap_simulated_annealing.c:12-206) for nruns
  * independent runs on X: run r uses the sign pattern of types[r] (H = ee^T
  * with the first types[r]-1 entries of e negated) and its own random stream

@@ -526,6 +526,9 @@ int admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax
 int admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax, int min_done);
 int admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
 
+// Resolve the no-allocation path's device array on `device` (current).
+int noalloc_setup(int device);
+
 // Triangle separation on the given slots (X current on the device).
 int tri_separate_slots(const std::vector<int>& slots, const int* K, double thresh,
                        std::vector<std::vector<madam_tri>>& lists, std::vector<double>& maxval);

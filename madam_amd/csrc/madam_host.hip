@@ -184,6 +184,8 @@ int madam_open(int device) {
     return MADAM_E_NODEV;
   }
   HIP_TRY(hipSetDevice(device));
+  // the no-allocation entry point's device array: its address on this device
+  if (int rc = noalloc_setup(device)) return rc;
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   Engine* e = new Engine();
