@@ -6,6 +6,12 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads (a "step" bounds the nodes listed; inputs are built before the
 timed region, instance data is uploaded inside each bound call):
+  c5        (default) branch and bound of the 8x8x8 toroidal grid (n = 512,
+            weights round(100 N(0,1))), --c5-nodes node bounds from the root
+            per step; plain and CPU legs on a shared sample of tree nodes,
+            the CPU leg on one core and on worker processes; the root's
+            last ADMM call replayed on identical inputs in the chosen and
+            plain forms and by the FP64 oracle (bound error at 1e-6)
   c2-root   root SDPbound of G(100, 0.3), integer weights uniform on
             +-{1..10}, seed 2 (IPM + cutting-plane loop with triangle,
             pentagonal and heptagonal cuts, params file values)
@@ -24,8 +30,9 @@ Forms:
             of a step, or on the first --plain-nodes of c3-batch
 The metric value is node bounds per second of the chosen form.
 `cpu_baseline` times the CPU FP64 oracle (LAPACK ADMM, dense Cholesky cut
-solve) on a bounded sample of the same nodes, one core.  `roofline` prices
-the projection kernel from its HIP-event time in a separate profiled step.
+solve) on a bounded sample of the same nodes (c5: on worker processes, one
+per available core up to 16, and on one core).  `roofline` prices the
+projection kernel from its HIP-event time in a separate profiled step.
 """
 from __future__ import annotations
 
@@ -279,37 +286,151 @@ def wl_bab150(args):
     return cfg, step, cpu
 
 
+C5_PARAMS = dict(TriIneq=3000, adjust_TriIneq=0, PentIneq=1000, HeptaIneq=1000)
+
+
+def _cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def _oracle_node(job):
+    """One oracle SDPbound (bounding.c:25-250) of a node, in a worker
+    process: (seconds, bound)."""
+    SPL, params, LB, diff, xf, sx = job
+    import ctypes as CC
+    from oracle import oracle_py as O
+    CC.CDLL(None).srand(2020)
+    t0 = time.perf_counter()
+    s = O.Solver(SPL, O.default_params(**params))
+    s.set_lb(LB)
+    s.s.diff = diff
+    node = s.load_subproblem(xf, sx)
+    b = s.evaluate(node, 1)
+    return time.perf_counter() - t0, b
+
+
 def wl_c5(args):
-    """Root SDPbound of the 8x8x8 toroidal grid (n=512, Gaussian weights
-    x100 rounded), tri+pent+hept cuts capped at 3000 + 1000 + 1000 per
-    round, chosen form; then --c5-nodes B&B nodes with a pipeline of 64.
-    The plain form (Jacobi on n = 512, thousands of ADMM iterations) and the
-    CPU oracle are not run: each would take many minutes per bound."""
+    """Config 5 at scale: branch and bound of the 8x8x8 toroidal grid (n =
+    512, weights round(100 N(0,1)), seed 5), tri + pent + hept cuts capped at
+    3000 + 1000 + 1000 per round.  A step = the search from the root with a
+    budget of --c5-nodes node bounds (root included), pipeline of 64 nodes,
+    chosen form; value = node bounds per second over the step.  The search
+    is far from proven at that budget (the root gap is ~2%); the line
+    reports the root bound, incumbent and gap.
+    Comparison legs on a shared sample (--plain-nodes nodes at the top of
+    the tree: 1-4 random fixings, the step's incumbent and diff, each node
+    pruning against its own heuristic's incumbent): the chosen form in one
+    batch, the plain FP64 form one node at a time, and the FP64 oracle on
+    one core and on P worker processes.  Bound error at identical inputs:
+    the root's last ADMM call (its cut set, cut duals and warm start,
+    captured by madam_call_state) run again in the chosen and plain forms
+    and by the FP64 oracle."""
     import madam_amd as G
     from madam_amd import instances as I
     SPL = I.objective_matrix(*I.torus3d(8, seed=5))
-    params = dict(TriIneq=3000, adjust_TriIneq=0, PentIneq=1000, HeptaIneq=1000)
+    N = SPL.shape[0]
+    sample = I.random_fixings(N - 1, args.plain_nodes, seed=55, kmin=1, kmax=4)
+    state = {}
 
-    def step(form, sample=None):
-        B = G.Bounder(SPL, G.default_params(**params), verbose=1, **CHOSEN)
-        b = B.sdpbound([B.node()], ranks=[0])[0]
-        out = {"bound": float(b), "bounds": np.array([b]), "admm_iters": int(B.ctx.admm_iters),
-               "nodes": 1, "n3_iters": float(B.ctx.admm_iters) * 512 ** 3,
-               "incumbent": float(B.ctx.LB)}
-        if args.c5_nodes > 0:
+    def step(form, sample_n=None):
+        if form == "chosen":
+            B = G.Bounder(SPL, G.default_params(**C5_PARAMS), batch=64, **CHOSEN)
+            B.bab(max_nodes=args.c5_nodes)
+            state.update(LB=float(B.ctx.LB), diff=float(B.ctx.diff),
+                         root_bound=float(B.ctx.root_bound))
+            return {"bound": float(B.ctx.root_bound), "bounds": np.array([B.ctx.root_bound]),
+                    "admm_iters": int(B.ctx.admm_iters), "nodes": int(B.ctx.nodes),
+                    "n3_iters": float(B.ctx.admm_iters) * N ** 3, "n": N,
+                    "incumbent": float(B.ctx.LB), "open": int(B.ctx.open_nodes),
+                    "admm_calls": int(B.ctx.admm_calls), "ipm_iters": int(B.ctx.ipm_iters),
+                    "ns_fallbacks": int(B.ctx.ns_fallbacks)}
+        # plain FP64 form, one node at a time, on the shared sample
+        bs, it = [], 0
+        for q, (xf, sx) in enumerate(sample):
+            B = G.Bounder(SPL, G.default_params(**C5_PARAMS), LB=state["LB"], own_incumbent=1,
+                          **PLAIN)
+            B.ctx.diff = state["diff"]
+            bs.append(B.sdpbound([B.node(xf, sx)], ranks=[1])[0])
+            it += int(B.ctx.admm_iters)
+            log(f"plain: sample node {q + 1} of {len(sample)} bounded")
+        return {"bound": float(np.mean(bs)), "bounds": np.array(bs), "admm_iters": it,
+                "nodes": len(sample), "n3_iters": float(it) * N ** 3}
+
+    def chosen_on_sample():
+        t0 = time.perf_counter()
+        B = G.Bounder(SPL, G.default_params(**C5_PARAMS), LB=state["LB"], own_incumbent=1,
+                      **CHOSEN)
+        B.ctx.diff = state["diff"]
+        bs = B.sdpbound([B.node(xf, sx) for xf, sx in sample], ranks=[1] * len(sample))
+        dt = time.perf_counter() - t0
+        return {"value": len(sample) / dt, "seconds": dt, "bounds": [float(b) for b in bs]}
+
+    def cpu(seconds, pool=None):
+        jobs = [(SPL, C5_PARAMS, state["LB"], state["diff"], xf, sx) for xf, sx in sample]
+        t0 = time.perf_counter()
+        one = [_oracle_node(jobs[0])]
+        t1 = time.perf_counter() - t0
+        out = {"one_core": {"value": 1.0 / t1, "seconds": t1, "nodes": 1,
+                            "bound": float(one[0][1])}}
+        if pool is not None:
             t0 = time.perf_counter()
-            B2 = G.Bounder(SPL, G.default_params(**params), batch=64, **CHOSEN)
-            B2.bab(max_nodes=args.c5_nodes)
-            out["bab"] = {"nodes": int(B2.ctx.nodes), "seconds": time.perf_counter() - t0,
-                          "incumbent": float(B2.ctx.LB), "open": int(B2.ctx.open_nodes)}
+            res = pool.map(_oracle_node, jobs, chunksize=1)
+            dt = time.perf_counter() - t0
+            out["workers"] = {"value": len(jobs) / dt, "seconds": dt, "nodes": len(jobs),
+                              "bounds": [float(b) for _, b in res],
+                              "node_seconds": [float(t) for t, _ in res]}
         return out
 
-    def cpu(seconds):
-        return None, "not timed for this workload", None
+    def replay():
+        """The root's last ADMM call on identical inputs in three arithmetics."""
+        from oracle import oracle_py as O
+        B = G.Bounder(SPL, G.default_params(**C5_PARAMS), **CHOSEN)
+        t0 = time.perf_counter()
+        b, prob = B.sdpbound_last_call(B.node(), rank=0)
+        t_root = time.perf_counter() - t0
+        calls = prob.pop("calls")
+        out = {"root_seconds": t_root, "root_bound": float(b), "call": calls,
+               "n": prob["L"].shape[0], "tol": prob["tol"],
+               "cuts": [len(prob["tri"]), len(prob["pent"]), len(prob["hept"])]}
+        for name, opts in (("chosen", dict(projection=1, cg_rtol=1e-7, cg_adapt=0.01)),
+                           ("plain", dict(projection=0))):
+            t0 = time.perf_counter()
+            r = G.ADMM_solver(**prob, **opts)
+            out[name] = {"f": r.f, "nbit": r.nbit, "seconds": time.perf_counter() - t0}
+            log(f"replay: {name} form done")
+        # the FP64 oracle (LAPACK dsyevr projection) on the same inputs
+        p = O.default_params(**C5_PARAMS)
+        s = O.Solver(SPL, p)
+        s.load_subproblem(None, None)
+        s.set_cuts(prob["tri"], prob["pent"], prob["hept"])
+        n = prob["L"].shape[0]
+        s.mat("X")[:] = prob["X"]
+        s.mat("Q")[:] = prob["Q"]
+        s.vec("y", n)[:] = prob["y"]
+        t0 = time.perf_counter()
+        f, _, nbit = s.admm(prob["sigma"], prob["tol"])
+        out["oracle"] = {"f": f, "nbit": nbit, "seconds": time.perf_counter() - t0}
+        log("replay: oracle done")
+        for name in ("chosen", "plain"):
+            out[name]["rel_err_vs_oracle"] = abs(out[name]["f"] - f) / abs(f)
+        out["chosen_rel_err_vs_plain"] = abs(out["chosen"]["f"] - out["plain"]["f"]) / abs(
+            out["plain"]["f"])
+        return out
 
-    cfg = {"workload": "root SDPbound of the 8x8x8 torus (n=512), Gaussian weights x100, "
-                       "<= 5000 tri/pent/hept cuts per round", "n": 512, "nodes_per_step": 1}
-    return cfg, step, cpu
+    cfg = {"workload": f"branch and bound of the 8x8x8 torus (n=512, weights round(100 N(0,1)), "
+                       f"seed 5) from the root, {args.c5_nodes} node bounds per step, pipeline of "
+                       f"64, <= 3000 + 1000 + 1000 tri/pent/hept cuts per round",
+           "n": N, "nodes_per_step": args.c5_nodes}
+    return cfg, step, cpu, {"chosen_on_sample": chosen_on_sample, "replay": replay,
+                            "sample": f"{len(sample)} nodes with 1-4 random fixings (seed 55)"}
 
 
 def wl_ns_batch(args):
@@ -344,8 +465,8 @@ def wl_ns_batch(args):
     return cfg, step, cpu
 
 
-WORKLOADS = {"c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1, "c4-learned": wl_c4,
-             "bab150": wl_bab150, "ns-batch": wl_ns_batch, "c5-root": wl_c5}
+WORKLOADS = {"c5": wl_c5, "c2-root": wl_c2, "c3-batch": wl_c3, "c1-bab": wl_c1,
+             "c4-learned": wl_c4, "bab150": wl_bab150, "ns-batch": wl_ns_batch}
 
 
 def main():
@@ -353,17 +474,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2-root", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-replay", action="store_true", help="c5: skip the identical-inputs replay")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="c5: oracle worker processes (0: the cores this process may use, "
+                         "at most 16)")
     ap.add_argument("--c4-train", type=int, default=2000, help="c4-learned: training graphs")
     ap.add_argument("--bab-nodes", type=int, default=20000,
                     help="bab150: nodes per step (0: solve to the proven optimum)")
-    ap.add_argument("--plain-nodes", type=int, default=4,
-                    help="c3-batch: nodes bounded one at a time by the plain form")
-    ap.add_argument("--c5-nodes", type=int, default=0, help="c5-root: B&B nodes after the root")
+    ap.add_argument("--plain-nodes", type=int, default=None,
+                    help="c3-batch: nodes bounded one at a time by the plain form (4); c5: "
+                         "nodes of the shared comparison sample (8)")
+    ap.add_argument("--c5-nodes", type=int, default=63,
+                    help="c5: node bounds per step (root included)")
     ap.add_argument("--ns-n", type=int, default=150, help="ns-batch: node order")
     ap.add_argument("--ns-k", type=int, default=128, help="ns-batch: nodes in the batch")
     ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2],
@@ -373,16 +500,30 @@ def main():
                     help="form run in the warmup, timed and profiled steps (plain: to profile "
                          "the plain FP64 form alone; the line then reports it as value)")
     args = ap.parse_args()
+    if args.plain_nodes is None:
+        args.plain_nodes = 8 if args.workload == "c5" else 4
     CHOSEN["ns_layout"] = args.ns_layout
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the bounding path "
               "shards by B&B node across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline's worker processes are forked before anything touches
+    # the GPU
+    cpu_model, cpu_avail = _cpu_info()
+    pool = None
+    if args.workload == "c5" and not args.no_cpu:
+        import multiprocessing as mp
+        nw = args.cpu_workers if args.cpu_workers > 0 else min(16, cpu_avail)
+        pool = mp.get_context("fork").Pool(nw)
+        pool_size = nw
+
     import madam_amd as G
     G.open_device(0)
     lib = G.lib()
-    cfg, step, cpu = WORKLOADS[args.workload](args)
+    wl = WORKLOADS[args.workload](args)
+    cfg, step, cpu = wl[:3]
+    extra = wl[3] if len(wl) > 3 else {}
 
     for w in range(args.warmup):
         step(args.timed_form)
@@ -410,15 +551,14 @@ def main():
         sec = eig_ms * 1e-3
         achieved = (f32 + f64) / sec / 1e12 if sec > 0 else 0.0
         t_peak = (f32 / PEAK_FP32_MFMA + f64 / PEAK_FP64_MFMA) / 1e12
-        # HBM bytes per projection, measured for this workload with rocprofv3
-        # FETCH_SIZE / WRITE_SIZE (per-dispatch KiB scaled by a 64 MiB copy:
-        # fetch x2.00, write x1.00), summed over the launches of one
-        # projection (one node of order 100: tile-parallel layout, 39 x
-        # k_ns_gemm32 135.5 KiB + 3 x k_ns_gemm64 213.6 + k_ns_init 123.5 +
-        # k_ns_conv 132.4) and scaled to the projections that did work
-        # (798 of the 864 timed launches): the FP32 iterates pass through L2
-        # between launches.  Algorithmic: W in, X+ out, 2 x 8 n^2 bytes.
-        traffic = 6692.0 * 1024 if args.workload == "c2-root" else None
+        # HBM bytes per projection, measured with rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE (per-dispatch KiB scaled by a 64 MiB copy: fetch x2.00,
+        # write x1.00), summed over the launches of one projection.  One
+        # node of order 100 (tile-parallel layout): 6692 KiB -- the FP32
+        # iterates pass through L2 between launches.  Algorithmic: W in, X+
+        # out, 2 x 8 n^2 bytes.
+        traffic = {"c2-root": 6692.0 * 1024}.get(args.workload)
+        phase_ms = {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}
         roofline = {"bound": "mfma", "kernel": "sign-iteration projection (39 FP32-MFMA products + 3 "
                     "FP64-MFMA products; k_ns_gemm32/k_ns_gemm64 tile launches or k_ns_project)",
                     "achieved": achieved,
@@ -431,10 +571,12 @@ def main():
                     "traffic_unit": "HBM bytes per projection",
                     "algorithmic_bytes_per_projection": 16.0 * rp.get("n", 0) ** 2 or None,
                     "projections": rp["admm_iters"],
-                    "phase_ms": {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}}
+                    "phase_ms": phase_ms,
+                    "phase_share": {p: round(v / max(1e-9, sum(phase_ms.values())), 4)
+                                    for p, v in phase_ms.items()}}
 
     plain = None
-    if not args.no_plain and args.workload != "c5-root":
+    if not args.no_plain:
         sample = args.plain_nodes if args.workload == "c3-batch" else None
         t1 = time.perf_counter()
         pr = step("plain", sample)
@@ -442,9 +584,17 @@ def main():
         log("plain form done")
         nb = len(pr["bounds"])
         plain = {"value": pr["nodes"] / tp, "unit": "node bounds/s", "seconds": tp,
-                 "nodes": pr["nodes"], "bound": pr["bound"], "admm_iters": pr["admm_iters"],
-                 "chosen_over_plain": value / (pr["nodes"] / tp)}
-        diff = np.abs(r["bounds"][:nb] - pr["bounds"]) / np.abs(pr["bounds"])
+                 "nodes": pr["nodes"], "bound": pr["bound"], "admm_iters": pr["admm_iters"]}
+        if args.workload == "c5":
+            cs = extra["chosen_on_sample"]()
+            log("chosen form on the sample done")
+            plain["sample"] = extra["sample"] + ", one at a time"
+            plain["chosen_same_sample"] = cs
+            plain["chosen_over_plain"] = cs["value"] / plain["value"]
+            diff = np.abs(np.array(cs["bounds"]) - pr["bounds"]) / np.abs(pr["bounds"])
+        else:
+            plain["chosen_over_plain"] = value / plain["value"]
+            diff = np.abs(r["bounds"][:nb] - pr["bounds"]) / np.abs(pr["bounds"])
         plain["rel_bound_diff_max"] = float(diff.max())
         plain.update(pr.get("extra", {}))
         if "admm_calls" in pr:
@@ -454,12 +604,30 @@ def main():
 
     cpub = None
     if not args.no_cpu and args.workload != "c4-learned":
-        cval, sample, cbound = cpu(args.cpu_seconds)
-        cpub = {"value": cval, "unit": "node bounds/s", "cores": 1, "kind": "port",
-                "sample": sample + " (1 thread, MKL_NUM_THREADS=1)"}
-        if cbound is not None:
-            cpub["bound"] = float(cbound)
-            cpub["rel_bound_diff_chosen_vs_oracle"] = abs(r["bound"] - cbound) / abs(cbound)
+        if args.workload == "c5":
+            co = cpu(args.cpu_seconds, pool)
+            w = co["workers"]
+            cpub = {"value": w["value"], "unit": "node bounds/s", "cores": pool_size,
+                    "kind": "port", "cpu_model": cpu_model, "cores_available": cpu_avail,
+                    "sample": f"{extra['sample']} bounded by the FP64 oracle (LAPACK ADMM, "
+                              f"MKL_NUM_THREADS=1) on {pool_size} worker processes, "
+                              f"{w['seconds']:.1f} s",
+                    "one_core": co["one_core"], "workers": w}
+            if plain is not None:
+                cs = plain["chosen_same_sample"]
+                cpub["chosen_over_workers"] = cs["value"] / w["value"]
+                cpub["rel_bound_diff_chosen_vs_oracle"] = float(np.max(
+                    np.abs(np.array(cs["bounds"]) - np.array(w["bounds"])) /
+                    np.abs(np.array(w["bounds"]))))
+            pool.close()
+        else:
+            cval, sample, cbound = cpu(args.cpu_seconds)
+            cpub = {"value": cval, "unit": "node bounds/s", "cores": 1, "kind": "port",
+                    "cpu_model": cpu_model, "cores_available": cpu_avail,
+                    "sample": sample + " (1 thread, MKL_NUM_THREADS=1)"}
+            if cbound is not None:
+                cpub["bound"] = float(cbound)
+                cpub["rel_bound_diff_chosen_vs_oracle"] = abs(r["bound"] - cbound) / abs(cbound)
 
     line = {"metric": METRIC, "value": value, "unit": "node bounds/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -469,9 +637,14 @@ def main():
             "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
     if "admm_calls" in r:
         line["chosen"]["admm_calls"] = r["admm_calls"]
-    if args.workload == "c5-root":
+    if args.workload == "c5":
         line["c5"] = {"root_bound": r["bound"], "incumbent": r["incumbent"],
-                      "gap": (r["bound"] - r["incumbent"]) / r["incumbent"], "bab": r.get("bab")}
+                      "gap": (r["bound"] - r["incumbent"]) / r["incumbent"],
+                      "nodes_per_step": r["nodes"], "open_nodes": r["open"],
+                      "ns_fallbacks": r["ns_fallbacks"], "ipm_iters": r["ipm_iters"],
+                      "proven": r["open"] == 0}
+        if not args.no_replay:
+            line["bound_error_identical_inputs"] = extra["replay"]()
     if args.workload == "c1-bab" or (args.workload == "bab150" and r.get("open") == 0):
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]

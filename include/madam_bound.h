@@ -222,6 +222,25 @@ typedef struct {
     double upper_bound;
 } madam_node;
 
+/* The inputs of a node's latest ADMM_solver call inside SDPbound (no
+ * reference counterpart; the reference keeps them in globals between the
+ * calls of bounding.c:121-236): the cut lists with the duals the call
+ * starts from (admm.c:79-95), the warm start X, y, Q, sigma and the
+ * tolerance, so that the same call can be run again in another form
+ * (madam_admm_batch) on identical inputs.  The caller sets the capacities
+ * and buffers: X, Q cap_n x cap_n, y cap_n, the lists cap_tri/cap_pent/
+ * cap_hept entries.  L is the node's subproblem (madam_create_subproblem). */
+typedef struct {
+    int cap_n, cap_tri, cap_pent, cap_hept;   /* in */
+    double *X, *y, *Q;                        /* out buffers */
+    madam_tri *tri;
+    madam_pent *pent;
+    madam_hept *hept;
+    int n, ntri, npent, nhept;                /* out */
+    double sigma, tol;                        /* out */
+    int calls;                                /* out: ADMM calls of the node up to this one */
+} madam_call_state;
+
 /* What the reference keeps in process globals for bounding and branching:
  * the instance (SP), the parameters, the incumbent (heap.c:8-9), diff
  * (global_var.h:13), plus this library's random-stream state and counters. */
@@ -253,6 +272,8 @@ typedef struct {
     double cg_adapt;                /* as madam_opts.cg_adapt */
     int gw_mult;                    /* hyperplanes per rounding-heuristic call: gw_mult x N
                                      * (<= 1: N, as heuristic.c) */
+    madam_call_state *last_call;    /* optional (null: off): madam_sdpbound_batch writes the
+                                     * inputs of its first node's latest ADMM call here */
 } madam_bab_ctx;
 
 /* createSubproblem + getFixedValue (evaluate.c:39-139): PPL gets the n x n

@@ -376,6 +376,17 @@ class BabCtx(C.Structure):
         ("seconds_heur", C.c_double), ("cg_rtol", C.c_double), ("cg_iters", C.c_longlong),
         ("ns_fallbacks", C.c_longlong), ("verbose", C.c_int), ("ns_layout", C.c_int),
         ("own_incumbent", C.c_int), ("cg_adapt", C.c_double), ("gw_mult", C.c_int),
+        ("last_call", C.c_void_p),
+    ]
+
+
+class CallState(C.Structure):
+    """madam_call_state: the inputs of a node's latest ADMM call."""
+    _fields_ = [
+        ("cap_n", C.c_int), ("cap_tri", C.c_int), ("cap_pent", C.c_int), ("cap_hept", C.c_int),
+        ("X", _PD), ("y", _PD), ("Q", _PD), ("tri", C.POINTER(Tri)), ("pent", C.POINTER(Pent)),
+        ("hept", C.POINTER(Hept)), ("n", C.c_int), ("ntri", C.c_int), ("npent", C.c_int),
+        ("nhept", C.c_int), ("sigma", C.c_double), ("tol", C.c_double), ("calls", C.c_int),
     ]
 
 
@@ -473,6 +484,35 @@ class Bounder:
         _check(lib().madam_sdpbound_batch(C.byref(self.ctx), C.c_int(k), arr, rk.ctypes.data_as(_PI),
                                           out.ctypes.data_as(_PD)))
         return out
+
+    def sdpbound_last_call(self, node, rank=0, cap_cuts=50000):
+        """SDPbound of one node that also returns the inputs of its last
+        ADMM_solver call (madam_call_state) as an ADMM_solver problem dict:
+        the subproblem L, the warm start X, y, Q, sigma, tol and the cut
+        lists with the duals the call started from."""
+        N = self.ctx.N
+        X, Q, y = np.zeros(N * N), np.zeros(N * N), np.zeros(N)
+        tri, pent, hept = (Tri * cap_cuts)(), (Pent * cap_cuts)(), (Hept * cap_cuts)()
+        st = CallState(cap_n=N, cap_tri=cap_cuts, cap_pent=cap_cuts, cap_hept=cap_cuts)
+        st.X, st.y, st.Q = (a.ctypes.data_as(_PD) for a in (X, y, Q))
+        st.tri = C.cast(tri, C.POINTER(Tri))
+        st.pent = C.cast(pent, C.POINTER(Pent))
+        st.hept = C.cast(hept, C.POINTER(Hept))
+        self.ctx.last_call = C.cast(C.pointer(st), C.c_void_p)
+        try:
+            b = self.sdpbound([node], ranks=[rank])[0]
+        finally:
+            self.ctx.last_call = None
+        n = st.n
+        L, _ = create_subproblem(self.SPL, list(node.xfixed)[:N - 1], list(node.solX)[:N - 1])
+        return b, dict(
+            L=L, X=np.asfortranarray(X[:n * n].reshape(n, n, order="F")), y=y[:n].copy(),
+            Q=np.asfortranarray(Q[:n * n].reshape(n, n, order="F")),
+            tri=[(c.type, c.i, c.j, c.k, c.y) for c in tri[:st.ntri]],
+            pent=[(c.type, tuple(c.perm), c.y) for c in pent[:st.npent]],
+            hept=[(c.type, tuple(c.perm), c.y) for c in hept[:st.nhept]],
+            sigma=st.sigma, tol=st.tol, itermax=self.ctx.params.ADMM_itermax,
+            scaleSigma=self.ctx.params.scaleSigma, calls=st.calls)
 
     def node(self, xfixed=None, solX=None):
         x = Node()

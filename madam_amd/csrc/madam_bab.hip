@@ -104,6 +104,7 @@ struct Run {
   double sigma = 1, tol = 0.05, viol3 = 0, viol5 = 0;
   double lb = 0;  // this node's incumbent when ctx->own_incumbent is set
   int count = 0, nbit = 0, done = 0, giveup = 0, prune = 0;
+  bool capture = false;  // write each ADMM call's inputs to ctx->last_call
   std::vector<double> PPL;
   std::vector<madam_tri> tri;
   std::vector<madam_pent> pent;
@@ -351,6 +352,49 @@ madam_opts bab_opts(const madam_bab_ctx* c) {
   return o;
 }
 
+// The inputs of the ADMM call run r is about to start -- X, y, Q in its
+// slot (the previous call's or the IPM's iterates), the cut lists with the
+// duals the call starts from, sigma and tol -- into ctx->last_call.
+int capture_call(madam_bab_ctx* c, Run& r, Slot& sl, const CutSet& cs) {
+  madam_call_state& S = *c->last_call;
+  if (S.cap_n < r.n || S.cap_tri < (int)r.tri.size() || S.cap_pent < (int)r.pent.size() ||
+      S.cap_hept < (int)r.hept.size() || !S.X || !S.y || !S.Q ||
+      (!r.tri.empty() && !S.tri) || (!r.pent.empty() && !S.pent) || (!r.hept.empty() && !S.hept)) {
+    set_err("last_call: buffers too small for n=%d and %zu/%zu/%zu cuts", r.n, r.tri.size(),
+            r.pent.size(), r.hept.size());
+    return MADAM_E_ARG;
+  }
+  Engine& E = *g_eng;
+  const size_t nn = (size_t)r.n * r.n;
+  int rc;
+  if ((rc = down(S.X, sl.X.p, nn, E.stream)) || (rc = down(S.Q, sl.Q.p, nn, E.stream)) ||
+      (rc = down(S.y, sl.y.p, (size_t)r.n, E.stream)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  // the duals the call starts from are the cut structs' y (build_cutset's y0)
+  const int nt = (int)r.tri.size(), np = (int)r.pent.size();
+  for (int q = 0; q < nt; ++q) {
+    S.tri[q] = r.tri[q];
+    S.tri[q].y = cs.y0[q];
+  }
+  for (int q = 0; q < np; ++q) {
+    S.pent[q] = r.pent[q];
+    S.pent[q].y = cs.y0[nt + q];
+  }
+  for (size_t q = 0; q < r.hept.size(); ++q) {
+    S.hept[q] = r.hept[q];
+    S.hept[q].y = cs.y0[nt + np + q];
+  }
+  S.n = r.n;
+  S.ntri = nt;
+  S.npent = np;
+  S.nhept = (int)r.hept.size();
+  S.sigma = r.sigma;
+  S.tol = r.tol;
+  S.calls = r.count;
+  return MADAM_OK;
+}
+
 // Start one ADMM_solver call (admm.c:49-771) per run: its cut structure,
 // dual warm start and a fresh view join the calls in flight.
 int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
@@ -391,6 +435,7 @@ int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
         return rc;
       HIP_TRY(hipMemsetAsync(sl.u.p, 0, cs[q].m * sizeof(double), st));
     }
+    if (r.capture && c->last_call && (rc = capture_call(c, r, sl, cs[q]))) return rc;
     NodeView v = E.hviews[r.slot];
     NodeView fresh;
     bind_view(fresh, sl, r.n, cs[q]);
@@ -676,6 +721,7 @@ int sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, const int*
     init_run(c, runs[q], nodes[q], ranks ? ranks[q] : 1, q);
     all.push_back(&runs[q]);
   }
+  runs[0].capture = c->last_call != nullptr;
   if ((rc = start_runs(c, all))) return rc;
   Rounds R;
   for (Run* r : all)

@@ -134,7 +134,7 @@ int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, in
     if (o.projection == 1 && madam::safe_hh_lds_bytes(nmax) + 1024 <= E.lds_limit)
       madam::launch_safe_hh(E.fviews.p, k, nmax, st);
     else if (o.projection == 1)
-      madam::launch_safe_lc(E.fviews.p, k, nmax, st);
+      madam::launch_safe_hhg(E.fviews.p, k, nmax, st);
     else
       madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
     madam::launch_safe_bound(E.fviews.p, k, st);
@@ -420,7 +420,7 @@ int madam_lambda_max_cert(int n, const double* A, double* lam) {
   if (madam::safe_hh_lds_bytes(n) + 1024 <= g_eng->lds_limit)
     madam::launch_safe_hh(g_eng->views.p, 1, n, st);
   else
-    madam::launch_safe_lc(g_eng->views.p, 1, n, st);
+    madam::launch_safe_hhg(g_eng->views.p, 1, n, st);
   madam::launch_safe_bound(g_eng->views.p, 1, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;
