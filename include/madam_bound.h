@@ -160,9 +160,9 @@ int madam_ns_schedule(int cap, float *coef);
 int madam_lambda_max(int n, const double *A, double *lam);
 
 /* The same bound as the mixed-precision form computes it: an estimate
- * (Householder tridiagonalisation and Sturm multisection in LDS for n up to
- * about 195, Lanczos beyond) certified by an FP64 Cholesky factorisation of
- * mu I - A (lam >= lambda_max(A), within ~1e-10 |A|_F). */
+ * (Householder tridiagonalisation and Sturm multisection, in LDS for n up
+ * to about 195, in device memory beyond) certified by an FP64 Cholesky
+ * factorisation of mu I - A (lam >= lambda_max(A), within ~1e-10 |A|_F). */
 int madam_lambda_max_cert(int n, const double *A, double *lam);
 
 /* getViolated_TriangleInequalities (cutting_planes.c:48-127): the at most
@@ -280,6 +280,24 @@ typedef struct {
  * subproblem matrix (n = free vertices + 1), *fixedvalue the constant. */
 int madam_create_subproblem(int N, const double *SPL, const int *xfixed, const int *solX,
                             double *PPL, int *n_out, double *fixedvalue);
+
+/* createSubproblem (evaluate.c:39-116) for k nodes on the device: node q's
+ * fixings are xfixed[q*(N-1) ..] and solX[q*(N-1) ..]; its n_out[q] x
+ * n_out[q] matrix is written to PPL + q*N*N (leading dimension n_out[q]),
+ * bit-identical to madam_create_subproblem's.  The batched driver builds
+ * its nodes this way. */
+int madam_create_subproblem_batch(int N, const double *SPL, int k, const int *xfixed,
+                                  const int *solX, double *PPL, int *n_out);
+
+/* readData (process_input.c:182-312): the rudy file at path ("n m", then m
+ * lines "i j w" with 1-based vertices) into SP->L (N x N, N = n <= capN),
+ * the same construction: the Laplacian of vertices 1..n-1, its row sums in
+ * the last row and column, their total in the corner.  A repeated edge
+ * overwrites the earlier weight.  MADAM_E_ARG, with the reason in
+ * madam_last_error(), for an unreadable file, header or edge, a vertex out
+ * of range, or n outside 2..min(capN, MADAM_NMAX).  Host only: needs no
+ * GPU. */
+int madam_read_rudy(const char *path, int capN, int *N, double *SPL);
 
 /* SDPbound (bounding.c:25) for k nodes at once: ranks[q] = 0 marks the
  * root (bounding.c:111, 241); bounds[q] = the node's upper bound, also

@@ -172,7 +172,8 @@ def project_SDP_ns(W):
 
 
 def lambda_max_cert(A):
-    """Certified upper bound on lambda_max(A) by Lanczos + Cholesky."""
+    """Certified upper bound on lambda_max(A): a Householder/Sturm estimate
+    proved by a Cholesky factorisation (the chosen form's safe bound)."""
     A = _f64(A)
     lam = C.c_double(0)
     _check(lib().madam_lambda_max_cert(C.c_int(A.shape[0]), A.ctypes.data_as(_PD), C.byref(lam)))
@@ -406,6 +407,40 @@ def create_subproblem(SPL, xfixed, solX):
                                          C.byref(fv)))
     k = n.value
     return np.asfortranarray(out[:k * k].reshape(k, k, order="F")), fv.value
+
+
+def create_subproblem_batch(SPL, fixings):
+    """createSubproblem (evaluate.c:39-116) for several nodes on the device:
+    fixings = [(xfixed, solX), ...] -> [PP->L, ...]."""
+    SPL = _f64(SPL)
+    N = SPL.shape[0]
+    k = len(fixings)
+    xf = np.zeros((k, N - 1), np.int32)
+    sx = np.zeros((k, N - 1), np.int32)
+    for q, (a, b) in enumerate(fixings):
+        xf[q] = a
+        sx[q] = b
+    out = np.zeros(k * N * N)
+    n = np.zeros(k, np.int32)
+    _check(lib().madam_create_subproblem_batch(C.c_int(N), SPL.ctypes.data_as(_PD), C.c_int(k),
+                                               xf.ctypes.data_as(_PI), sx.ctypes.data_as(_PI),
+                                               out.ctypes.data_as(_PD), n.ctypes.data_as(_PI)))
+    res = []
+    for q in range(k):
+        m = int(n[q])
+        blk = out[q * N * N: q * N * N + m * m]
+        res.append(np.asfortranarray(blk.reshape(m, m, order="F")))
+    return res
+
+
+def read_rudy(path, capN=NMAX):
+    """readData (process_input.c:182-312) in the library: SP->L of a rudy file."""
+    SPL = np.zeros(capN * capN)
+    N = C.c_int(0)
+    _check(lib().madam_read_rudy(str(path).encode(), C.c_int(capN), C.byref(N),
+                                 SPL.ctypes.data_as(_PD)))
+    n = N.value
+    return np.asfortranarray(SPL[:n * n].reshape(n, n, order="F"))
 
 
 def ipm(L):

@@ -78,12 +78,15 @@ int create_subproblem(int N, const double* SPL, const int* xfixed, const int* so
   return n;
 }
 
-// getFixedValue (evaluate.c:124-139).
+// getFixedValue (evaluate.c:124-139): the same terms in the same order,
+// the loops over the fixed vertices only.
 double fixed_value(int N, const double* SPL, const int* xfixed, const int* solX) {
-  double v = 0.0;
+  std::vector<int> fx;
   for (int i = 0; i < N - 1; ++i)
-    for (int j = 0; j < N - 1; ++j)
-      if (xfixed[i] && xfixed[j]) v += SPL[j + (size_t)i * N] * solX[i] * solX[j];
+    if (xfixed[i]) fx.push_back(i);
+  double v = 0.0;
+  for (int i : fx)
+    for (int j : fx) v += SPL[j + (size_t)i * N] * solX[i] * solX[j];
   return v;
 }
 
@@ -105,7 +108,8 @@ struct Run {
   double lb = 0;  // this node's incumbent when ctx->own_incumbent is set
   int count = 0, nbit = 0, done = 0, giveup = 0, prune = 0;
   bool capture = false;  // write each ADMM call's inputs to ctx->last_call
-  std::vector<double> PPL;
+  std::vector<double> PPL;  // the subproblem when given by the caller (madam_ipm);
+                            // empty: built on the device from the node's fixings
   std::vector<madam_tri> tri;
   std::vector<madam_pent> pent;
   std::vector<madam_hept> hept;
@@ -518,6 +522,9 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   const int k = (int)runs.size();
   int rc, nmax = 2;
   std::vector<NodeView> hv(k);
+  std::vector<const int*> xf, sx;
+  std::vector<double*> dst;
+  std::vector<int> dq;
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
     Slot& sl = E.slots[r.slot];
@@ -525,12 +532,33 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     if ((rc = build_cutset(r.n, 0, nullptr, 0, nullptr, 0, nullptr, cs0)) ||
         (rc = sl.size_for(r.n, cs0)) || (rc = sl.size_ipm(r.n)))
       return rc;
-    if ((rc = up(sl.L.p, r.PPL.data(), r.PPL.size(), st))) return rc;
     bind_view(hv[q], sl, r.n, cs0);
-    double fro = 0.0;
-    for (double v : r.PPL) fro += v * v;
-    hv[q].normL = sqrt(fro);
+    if (r.PPL.empty()) {  // createSubproblem on the device, straight into the slot
+      xf.push_back(r.node->xfixed);
+      sx.push_back(r.node->solX);
+      dst.push_back(sl.L.p);
+      dq.push_back(q);
+    } else {
+      if ((rc = up(sl.L.p, r.PPL.data(), r.PPL.size(), st))) return rc;
+      double fro = 0.0;
+      for (double v : r.PPL) fro += v * v;
+      hv[q].normL = sqrt(fro);
+    }
     nmax = std::max(nmax, r.n);
+  }
+  if (!dq.empty()) {
+    std::vector<int> nd(dq.size());
+    std::vector<double> nrm(dq.size());
+    if ((rc = device_subproblems(c->N, (int)dq.size(), xf.data(), sx.data(), dst.data(), nd.data(),
+                                 nrm.data())))
+      return rc;
+    for (size_t t = 0; t < dq.size(); ++t) {
+      if (nd[t] != runs[dq[t]]->n) {
+        set_err("device subproblem of order %d, expected %d", nd[t], runs[dq[t]]->n);
+        return MADAM_E_HIP;
+      }
+      hv[dq[t]].normL = nrm[t];
+    }
   }
   if ((rc = E.views.ensure(k)) || (rc = up(E.views.p, hv.data(), k, st))) return rc;
   TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, E.lds_limit, st));
@@ -559,7 +587,9 @@ void init_run(madam_bab_ctx* c, Run& r, madam_node* x, int rank, int slot) {
   r.node = x;
   r.rank = rank;
   r.slot = slot;
-  r.n = create_subproblem(c->N, c->SPL, x->xfixed, x->solX, r.PPL);
+  int nfree = 0;
+  for (int i = 0; i < c->N - 1; ++i) nfree += !x->xfixed[i];
+  r.n = nfree + 1;  // PP->L itself is built on the device (ipm_round)
   r.fixedvalue = fixed_value(c->N, c->SPL, x->xfixed, x->solX);
   r.sigma = c->params.sigma0;
   r.tol = c->params.tol0;
@@ -846,6 +876,7 @@ int madam_sdpbound_batch(madam_bab_ctx* c, int k, madam_node* const* nodes, cons
     set_err("madam_sdpbound_batch: bad arguments");
     return MADAM_E_ARG;
   }
+  if ((rc = upload_spl(c->N, c->SPL))) return rc;
   return sdpbound_batch(c, k, nodes, ranks, bounds);
 }
 
@@ -856,6 +887,7 @@ int madam_bab(madam_bab_ctx* c, long long max_nodes, double* optimum) {
     set_err("madam_bab: bad arguments");
     return MADAM_E_ARG;
   }
+  if ((rc = upload_spl(c->N, c->SPL))) return rc;
   if (c->params.adjust_TriIneq) c->params.TriIneq = c->N * 10;
   // one shared incumbent in the tree search, whatever own_incumbent says
   struct Restore {

@@ -526,6 +526,14 @@ int admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax
 int admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax, int min_done);
 int admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
 
+// Subproblems on the device (madam_subp.hip): upload_spl puts SP->L in HBM
+// for the current call; device_subproblems builds PP->L of k nodes into
+// dst[q] (device, n x n) and returns their orders and Frobenius norms.
+int upload_spl(int N, const double* SPL);
+int device_subproblems(int N, int k, const int* const* xfixed, const int* const* solX,
+                       double* const* dst, int* n_out, double* norm_out);
+void release_spl();
+
 // Resolve the no-allocation path's device array on `device` (current).
 int noalloc_setup(int device);
 

@@ -217,6 +217,7 @@ void madam_close(void) {
   g_eng->tmpd.release();
   g_eng->postpart.release();
   g_eng->fviews.release();
+  release_spl();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   delete g_eng;

@@ -20,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libmadam_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmadam_ref.so")
+REF_HEUR_SO = os.path.join(HERE, "_ref", "libmadam_ref_heur.so")
 
 NMAX = 1024
 MAX_CUTS = 50000
@@ -117,7 +118,7 @@ def lib():
                      "mo_fixed_value", "mo_tri_separate", "mo_qap_sa",
                      "mo_pent_separate", "mo_hept_separate", "mo_one_opt",
                      "mo_gw_explicit", "mo_tri_update", "mo_pent_update",
-                     "mo_hept_update", "mo_evaluate_solution"):
+                     "mo_hept_update", "mo_evaluate_solution", "mo_gw"):
             getattr(L, name).restype = C.c_double
         L.mo_bab.argtypes = [C.POINTER(State), C.c_long]
         L.mo_pent_H.restype = _PI

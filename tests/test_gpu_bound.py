@@ -177,3 +177,29 @@ def test_sdpbound_batch_of_fixed_nodes_is_valid(projection, cg_rtol, cg_adapt):
         print(q, "batch", bounds[q], "alone", alone, "brute force", best)
         assert bounds[q] >= best - 1e-6
         assert abs(bounds[q] - alone) <= 5e-3 * max(1.0, abs(alone))
+
+
+@pytest.mark.parametrize("inst", ["torus15", "torus8x8x8", "g05_60.0"])
+def test_device_subproblems_bit_identical_to_oracle(inst):
+    """createSubproblem (evaluate.c:39-116) built on the device for a batch
+    of nodes -- as the batched driver builds its nodes -- equals the
+    oracle's PP->L bit for bit, also with no and with almost all vertices
+    fixed."""
+    from madam_amd import instances as I
+    if inst == "torus15":
+        SPL = I.objective_matrix(*I.torus2d(15, 15, seed=3))
+    elif inst == "torus8x8x8":
+        SPL = I.objective_matrix(*I.torus3d(8, seed=5))
+    else:
+        SPL = graph_L("g05_60.0")
+    N = SPL.shape[0]
+    fix = I.random_fixings(N - 1, 12, seed=9, kmin=0, kmax=N - 3)
+    fix.append((np.zeros(N - 1, np.int32), np.zeros(N - 1, np.int32)))
+    allbut = np.ones(N - 1, np.int32)
+    allbut[N // 3] = 0
+    fix.append((allbut, (np.arange(N - 1) % 2).astype(np.int32)))
+    got = G.create_subproblem_batch(SPL, fix)
+    for (xf, sx), P in zip(fix, got):
+        ref, _ = O.create_subproblem(SPL, xf, sx)
+        assert P.shape == ref.shape
+        assert np.array_equal(P, ref)

@@ -15,6 +15,8 @@ Pins established here (bit-exact unless a tolerance is written):
   qap simulated annealing         qap_simulated_annealing.c:12-206
   pentagonal / heptagonal lists   cutting_planes.c:224-605
   ipm_mc_pk                       ipm_mc_pk.c:28-347   (1e-10 relative)
+  mc_1opt                         heuristic.c:100-198  (oracle/_ref/libmadam_ref_heur.so)
+  GW_heuristic + update_best      heuristic.c:24-98, 204-222 on the glibc rand() stream
 Skipped when oracle/_ref is absent (it is built only where the reference tree
 exists).
 """
@@ -232,3 +234,115 @@ def test_ipm_matches_reference(path):
     assert abs(phio - phi.value) <= 1e-10 * abs(phi.value)
     assert np.abs(yo - y).max() <= 1e-8 * np.abs(y).max()
     assert np.abs(Xo.ravel(order="F") - X).max() <= 1e-8
+
+
+# ---- the rounding heuristic: oracle/_ref/libmadam_ref_heur.so is the
+# reference's heuristic.c with the evaluateSolution it calls
+# (bab_functions.c) and the globals it reads, nothing else
+
+class RefHeur:
+    def __init__(self, SPL):
+        self.L = C.CDLL(O.REF_HEUR_SO, mode=C.RTLD_LOCAL)
+        for name in ("GW_heuristic", "mc_1opt", "evaluateSolution", "runHeuristic"):
+            getattr(self.L, name).restype = C.c_double
+        self.N = SPL.shape[0]
+        self.SPLk = np.asfortranarray(SPL).ravel(order="F").copy()
+        self.SP = O.Ref.Problem()
+        self.SP.L = self.SPLk.ctypes.data_as(PD)
+        self.SP.n = self.N
+        C.c_void_p.in_dll(self.L, "SP").value = C.addressof(self.SP)
+        C.c_int.in_dll(self.L, "BabPbSize").value = self.N - 1
+
+    def problem(self, Lsub):
+        P = O.Ref.Problem()
+        keep = np.asfortranarray(Lsub).ravel(order="F").copy()
+        P.L = keep.ctypes.data_as(PD)
+        P.n = Lsub.shape[0]
+        return P, keep
+
+    def set_Z(self, Z):
+        self.Zk = np.asfortranarray(Z).ravel(order="F").copy()
+        C.c_void_p.in_dll(self.L, "Z").value = self.Zk.ctypes.data
+        C.c_int.in_dll(self.L, "M").value = Z.shape[1]
+
+
+ref_heur = pytest.mark.skipif(not os.path.exists(O.REF_HEUR_SO),
+                              reason="oracle/_ref/libmadam_ref_heur.so not built")
+
+
+def _instance(name):
+    from tests.helpers import graph_L
+    return graph_L(name)
+
+
+@ref_heur
+@pytest.mark.parametrize("name,nfix", [("g05_60.0", 0), ("g05_60.0", 17), ("w01_100.3", 30)])
+def test_mc_1opt_matches_reference(name, nfix):
+    """mc_1opt (heuristic.c:100-198): the same local optimum and value from
+    the same start, bit for bit, on instances with ties between gains (unit
+    weights) and with negative weights."""
+    SPL = _instance(name)
+    N = SPL.shape[0]
+    rng = np.random.default_rng(nfix + 5)
+    xf = np.zeros(N - 1, np.int32)
+    xf[rng.choice(N - 1, nfix, replace=False)] = 1
+    sx = (rng.random(N - 1) < 0.5).astype(np.int32) * xf
+    Lsub, _ = O.create_subproblem(SPL, xf, sx)
+    R = RefHeur(SPL)
+    P, keep = R.problem(Lsub)
+    for _ in range(12):
+        x0 = rng.choice([-1, 1], size=Lsub.shape[0]).astype(np.int32)
+        xr = x0.copy()
+        cr = R.L.mc_1opt(xr.ctypes.data_as(C.POINTER(C.c_int)), C.byref(P))
+        co, xo = O.one_opt(x0.copy(), Lsub)
+        assert cr == co
+        assert np.array_equal(xr, xo)
+
+
+@ref_heur
+@pytest.mark.parametrize("name,nfix,seed", [("g05_60.0", 0, 11), ("g05_60.0", 9, 12),
+                                            ("w01_100.3", 25, 13)])
+def test_gw_heuristic_same_stream_as_reference(name, nfix, seed):
+    """GW_heuristic (heuristic.c:24-98) with update_best and
+    evaluateSolution: on the same factor Z, node and glibc rand() stream,
+    the oracle's GW draws the same hyperplanes and returns the same best
+    value and cut, bit for bit."""
+    SPL = _instance(name)
+    N = SPL.shape[0]
+    rng = np.random.default_rng(seed)
+    node = O.Node()
+    idx = rng.choice(N - 1, nfix, replace=False)
+    for i in idx:
+        node.xfixed[i] = 1
+        node.solX[i] = int(rng.integers(0, 2))
+    xf = np.array(node.xfixed[:N - 1], np.int32)
+    sx = np.array(node.solX[:N - 1], np.int32)
+    Lsub, _ = O.create_subproblem(SPL, xf, sx)
+    n = Lsub.shape[0]
+    # a factor Z (n x M) of an X like ADMM's: the IPM's primal
+    X, _, _, _, _ = O.ipm(Lsub)
+    w, V = np.linalg.eigh(X)
+    keepc = w > 1e-8
+    Z = np.asfortranarray(V[:, keepc] * np.sqrt(w[keepc]))
+    R = RefHeur(SPL)
+    P, keep = R.problem(Lsub)
+    R.set_Z(Z)
+    num = N
+    xr = np.zeros(N - 1, np.int32)
+    libc.srand(seed)
+    br = R.L.GW_heuristic(C.byref(R.SP), C.byref(P), C.byref(node),
+                          xr.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(num))
+    s = O.Solver(SPL, O.default_params())
+    s.load_subproblem(xf, sx)
+    assert s.s.n == n
+    s.mat("Z", n)  # the state's Z holds n x M from its start, like the reference's
+    zbuf = np.ctypeslib.as_array(s.s.Z, shape=(n * n,))
+    zbuf[:n * Z.shape[1]] = Z.ravel(order="F")
+    s.s.M = Z.shape[1]
+    xo = np.zeros(N - 1, np.int32)
+    libc.srand(seed)
+    bo = O.lib().mo_gw(s.st, C.byref(node), xo.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(num))
+    assert br == bo
+    assert np.array_equal(xr, xo)
+    # and the value is the cut's objective (evaluateSolution)
+    assert R.L.evaluateSolution(xr.ctypes.data_as(C.POINTER(C.c_int))) == br
