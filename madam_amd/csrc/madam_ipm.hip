@@ -165,6 +165,145 @@ __device__ void chol_solve1(const double* P, int n, int n2, const double* b, dou
   __syncthreads();
 }
 
+// Solve (L L^T) x = b with the packed factor in device memory (orders
+// whose factor does not fit LDS), by wave 0 with the vector in LDS (xs, n
+// doubles; b and x may alias): forward sweep column-oriented, b_i -= L_ik
+// x_k below x_k = b_k / L_kk with column k read across the lanes
+// (contiguous); backward sweep x_k = (z_k - sum_{i>k} L_ik x_i) / L_kk with
+// the sum over column k (lanes' partial sums, then a wave reduction).  A
+// wave's LDS accesses complete in order, so no barrier is needed between
+// its steps.
+__device__ void chol_solve_lds(const double* P, int n, int n2, const double* b, double* x,
+                               double* xs) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    for (int i = lane; i < n; i += 64) xs[i] = b[i];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < n; ++k) {
+      const int cb = pk(k, k, n2) - k;  // column k: P[cb + i] = L(i, k), i >= k
+      const double xk = xs[k] / P[cb + k];
+      for (int i = k + 1 + lane; i < n; i += 64) xs[i] -= P[cb + i] * xk;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) xs[k] = xk;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      const int cb = pk(k, k, n2) - k;
+      double part = 0.0;
+      for (int i = k + 1 + lane; i < n; i += 64) part += P[cb + i] * xs[i];
+      const double sm = wave_sum(part);
+      const double xk = (xs[k] - sm) / P[cb + k];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) xs[k] = xk;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = lane; i < n; i += 64) x[i] = xs[i];
+  }
+  __syncthreads();
+}
+
+// Blocked right-looking Cholesky of the packed lower triangle P (order n,
+// leading order n2) in device memory, for orders whose factor does not fit
+// LDS.  Panels of CB = 16 columns: the panel's rows j0..n-1 are staged in
+// LDS (Lp, CB x ldp doubles, column k at Lp + k ldp), wave 0 factors them
+// (unblocked, one column at a time, no barriers: a wave's LDS accesses are
+// in order), the panel goes back to P, and every wave then applies the
+// panel's rank-CB update to lower 16 x 16 tiles of the trailing block with
+// FP64 MFMA (v_mfma_f64_16x16x4f64: A operand lane l -> (row l % 16, k
+// l / 16), B operand lane l -> (k l / 16, col l % 16), result register v of
+// lane l -> (row l / 16 + 4 v, col l % 16)), three barriers per panel.
+// Returns 0, or c + 1 for the first column c with a non-positive pivot --
+// the same value in every thread.
+constexpr int CB = 16;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+size_t chol_blocked_lds_doubles(int n) { return (size_t)CB * (((n + 15) & ~15) + 1); }
+
+__device__ int chol_blocked(double* P, int n, int n2, int* flag, double* Lp) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const int ldp = ((n + 15) & ~15) + 1;  // odd stride: the k groups of a tile read spread banks
+  __syncthreads();  // readers of flag in a previous call are done
+  if (tid == 0) flag[0] = 0;
+  for (int j0 = 0; j0 < n; j0 += CB) {
+    const int nb = n - j0 < CB ? n - j0 : CB, m = n - j0;
+    // stage the panel: Lp[k ldp + (r - j0)] = P(r, j0 + k), r >= j0 + k (else 0)
+    for (int f = tid; f < CB * m; f += blockDim.x) {
+      const int k = f / m, rr = f - k * m;
+      Lp[k * ldp + rr] = (k < nb && rr >= k) ? P[pk(j0 + rr, j0 + k, n2)] : 0.0;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      for (int k = 0; k < nb; ++k) {
+        double* col = Lp + k * ldp;
+        const double d = col[k];
+        if (!(d > 0.0)) {
+          if (lane == 0) flag[0] = j0 + k + 1;
+          break;  // uniform in the wave
+        }
+        const double sd = sqrt(d);
+        for (int rr = k + 1 + lane; rr < m; rr += 64) col[rr] /= sd;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) col[k] = sd;
+        for (int q = k + 1; q < nb; ++q) {
+          double* cq = Lp + q * ldp;
+          const double lq = col[q];
+          for (int rr = q + lane; rr < m; rr += 64) cq[rr] -= col[rr] * lq;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
+    if (flag[0]) return flag[0];
+    // the factored panel back to P
+    for (int f = tid; f < CB * m; f += blockDim.x) {
+      const int k = f / m, rr = f - k * m;
+      if (k < nb && rr >= k) P[pk(j0 + rr, j0 + k, n2)] = Lp[k * ldp + rr];
+    }
+    // trailing block rows/columns t0..n-1: lower 16 x 16 tiles, one per wave
+    const int t0 = j0 + nb;
+    if (t0 < n) {
+      const int T = (n - t0 + 15) >> 4;
+      const int ntile = T * (T + 1) / 2;
+      const int rl = lane & 15, kl = lane >> 4;
+      for (int t = wv; t < ntile; t += nw) {
+        int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+        if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        if (ti * (ti + 1) / 2 > t) --ti;
+        const int tj = t - ti * (ti + 1) / 2;
+        const int r0 = t0 + 16 * ti, c0 = t0 + 16 * tj;
+        f64x4 acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + kl + 4 * v, c = c0 + rl;
+          acc[v] = (r < n && c < n && r >= c) ? P[pk(r, c, n2)] : 0.0;
+        }
+#pragma unroll
+        for (int kq = 0; kq < CB / 4; ++kq) {
+          const int k = 4 * kq + kl;
+          const int ra = r0 + rl - j0, rb = c0 + rl - j0;  // rows of the panel (< ldp - 1)
+          const double a = (k < nb && r0 + rl < n) ? -Lp[k * ldp + ra] : 0.0;
+          const double b = (k < nb && c0 + rl < n) ? Lp[k * ldp + rb] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + kl + 4 * v, c = c0 + rl;
+          if (r < n && c < n && r >= c) P[pk(r, c, n2)] = acc[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return 0;
+}
+
 struct IpmOut {
   double* C;
   int n;
@@ -182,6 +321,22 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   double* tB = tA + 16 * 65;
   __shared__ int flag[2];
   double* P = P_LDS ? (tB + 16 * 65) : nd.ipP;
+  // Cholesky and solves with the packed factor P: in LDS one column at a
+  // time; in device memory (orders too large for LDS) blocked, with the
+  // panel after tB, and the solve's vector in the tA slab
+  double* Lp = tB + 16 * 65;
+  auto chol = [&]() {
+    if constexpr (P_LDS)
+      return chol_packed(P, n, n2, flag);
+    else
+      return chol_blocked(P, n, n2, flag, Lp);
+  };
+  auto solve = [&](const double* b, double* x) {
+    if constexpr (P_LDS)
+      chol_solve1(P, n, n2, b, x);
+    else
+      chol_solve_lds(P, n, n2, b, x, tA);
+  };
   const double* __restrict__ L = nd.L;
   double* X = nd.X;
   double* Z = nd.Q;
@@ -222,16 +377,16 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
     ++it;
     // ---- Zi = Z^{-1}: factor, then invert from the factor
     load_packed(P, n, n2, Z, 0.0, nullptr, nullptr);
-    if (chol_packed(P, n, n2, flag)) { status = 1; break; }
+    if (chol()) { status = 1; break; }
     potri_packed(P, n, n2, Zi, tA);
     // ---- predictor: dy1 = (Zi .* X) \ (-e)
     for (size_t f = tid; f < nn; f += WG) Mm[f] = Zi[f] * X[f];
     __syncthreads();
     load_packed(P, n, n2, Mm, 0.0, nullptr, nullptr);
-    if (chol_packed(P, n, n2, flag)) { status = 3; break; }
+    if (chol()) { status = 3; break; }
     for (int i = tid; i < n; i += WG) dy1[i] = -1.0;
     __syncthreads();
-    chol_solve1(P, n, n2, dy1, dy1);  // P keeps M's factor for the corrector
+    solve(dy1, dy1);  // P keeps M's factor for the corrector
     // dX1 = -Zi Diag(dy1) X - X, symmetrised
     for (size_t f = tid; f < nn; f += WG) {
       const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
@@ -263,7 +418,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       dy[i] = -acc + mu * Zi[(size_t)i * (n + 1)];
     }
     __syncthreads();
-    chol_solve1(P, n, n2, dy, dy);  // the same M: its factor is still in P
+    solve(dy, dy);  // the same M: its factor is still in P
     // dX2 = mu Zi - Zi (Diag(dy2) X + Diag(dy1) dX1)
     for (size_t f = tid; f < nn; f += WG) {
       const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
@@ -291,14 +446,14 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
     double ap = 1.0;
     for (int tries = 0; tries < 400; ++tries) {
       load_packed(P, n, n2, X, ap, dX, nullptr);
-      if (!chol_packed(P, n, n2, flag)) break;
+      if (!chol()) break;
       ap *= 0.8;
     }
     if (ap < 1.0) ap *= 0.95;
     double ad = 1.0;
     for (int tries = 0; tries < 400; ++tries) {
       load_packed(P, n, n2, Z, ad, nullptr, dy);
-      if (!chol_packed(P, n, n2, flag)) break;
+      if (!chol()) break;
       ad *= 0.8;
     }
     if (ad < 1.0) ad *= 0.95;
@@ -332,7 +487,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
 
 size_t ipm_lds_bytes(int n, bool packed) {
   size_t d = 64 + 2 * 16 * 65;
-  if (packed) d += (size_t)n * (n + 1) / 2;
+  d += packed ? (size_t)n * (n + 1) / 2 : chol_blocked_lds_doubles(n);
   return d * sizeof(double);
 }
 

@@ -52,6 +52,31 @@ def test_ipm_matches_oracle(spec):
     assert np.abs(yg - yo).max() <= 1e-6 * np.abs(yo).max()
 
 
+@pytest.mark.parametrize("spec", ["torus15", "torus8x8x8", "torus8x8x8_fixed"])
+def test_ipm_large_order_matches_oracle(spec):
+    """The interior-point method at orders whose factor lives in device
+    memory (n = 225 and 512; blocked Cholesky with FP64 MFMA updates): the
+    same iteration count and phi to 1e-9 as the oracle's ipm_mc_pk
+    (ipm_mc_pk.c:28-347, LAPACK), X and y to 1e-6."""
+    if spec == "torus15":
+        SPL = I.objective_matrix(*I.torus2d(15, 15, seed=3))
+        xf = np.zeros(SPL.shape[0] - 1, np.int32)
+        sx = xf
+    else:
+        SPL = I.objective_matrix(*I.torus3d(8, seed=5))
+        xf, sx = I.random_fixings(SPL.shape[0] - 1, 1, seed=4, kmin=0, kmax=0)[0]
+        if spec.endswith("fixed"):
+            xf, sx = I.random_fixings(SPL.shape[0] - 1, 1, seed=4, kmin=40, kmax=40)[0]
+    L, _ = O.create_subproblem(SPL, xf, sx)
+    Xg, yg, Zg, phig, itg = G.ipm(L)
+    Xo, yo, Zo, phio, ito = O.ipm(L)
+    print(spec, "n", L.shape[0], "iters", itg, ito, "phi", phig, phio)
+    assert itg == ito
+    assert abs(phig - phio) <= 1e-9 * abs(phio)
+    assert np.abs(Xg - Xo).max() <= 1e-6
+    assert np.abs(yg - yo).max() <= 1e-6 * np.abs(yo).max()
+
+
 def test_gw_rounding_matches_oracle():
     SPL = graph_L("g05_60.0")
     L, _ = O.create_subproblem(SPL, np.zeros(59, np.int32), np.zeros(59, np.int32))
