@@ -304,6 +304,155 @@ __device__ int chol_blocked(double* P, int n, int n2, int* flag, double* Lp) {
   return 0;
 }
 
+// A 16 x 16 lower tile t of a T x T tile grid: (ti, tj) with ti >= tj.
+__device__ __forceinline__ void lower_tile(int t, int& ti, int& tj) {
+  int a = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  if ((a + 1) * (a + 2) / 2 <= t) ++a;
+  if (a * (a + 1) / 2 > t) --a;
+  ti = a;
+  tj = t - a * (a + 1) / 2;
+}
+
+// A^{-1} = W^T W, W = L^{-1}, from the packed Cholesky factor L of A in
+// device memory (dpotri for orders beyond LDS).  W in place, blocks of CB
+// columns from the right, with the trailing block W22 = L22^{-1} already
+// in place:  W11 = L11^{-1} (wave 0, in LDS: column c from the right,
+// W11(r, c) = -(1/L_cc) sum_{c<l<=r} W11(r, l) L(l, c), as dtrti2),
+// S = L21 W11 (each thread one row of the panel, in place in LDS), then
+// W21 = -W22 S on FP64 MFMA tiles (16 rows x the block's columns; W22 read
+// from device memory, masked to its lower triangle, S from LDS).  Then
+// out = W^T W (full, leading dimension n) as lower 16 x 16 MFMA tiles,
+// out(i, j) = sum_{k >= max(i, j)} W(k, i) W(k, j), mirrored.
+// LDS: Lp (CB x ldp doubles, the panel) and W11 (CB x CB, in tB).
+__device__ void potri_blocked(double* P, int n, int n2, double* out, double* Lp, double* W11) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const int ldp = ((n + 15) & ~15) + 1;
+  const int rl = lane & 15, kl = lane >> 4;
+  const int nblk = (n + CB - 1) / CB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int j0 = bi * CB, nb = n - j0 < CB ? n - j0 : CB, j1 = j0 + nb, m2 = n - j1;
+    // stage L11 (lower, nb x nb) into W11 and L21 (m2 x nb) into the panel
+    for (int f = tid; f < CB * CB; f += blockDim.x) {
+      const int c = f / CB, r = f - c * CB;
+      W11[c * CB + r] = (c < nb && r < nb && r >= c) ? P[pk(j0 + r, j0 + c, n2)] : 0.0;
+    }
+    for (int f = tid; f < CB * m2; f += blockDim.x) {
+      const int k = f / m2, rr = f - k * m2;
+      Lp[k * ldp + rr] = k < nb ? P[pk(j1 + rr, j0 + k, n2)] : 0.0;
+    }
+    __syncthreads();
+    if (wv == 0) {  // W11 = L11^{-1}, columns from the right
+      for (int c = nb - 1; c >= 0; --c) {
+        const double lcc = W11[c * CB + c];
+        double v = 0.0;
+        const int r = c + 1 + lane;  // lanes 0 .. nb-c-2 each own a row below c
+        if (r < nb) {
+          for (int l = c + 1; l <= r; ++l) v += W11[l * CB + r] * W11[c * CB + l];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (r < nb) W11[c * CB + r] = -v / lcc;
+        if (lane == 0) W11[c * CB + c] = 1.0 / lcc;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
+    // S = L21 W11, row by row in place: S(r, c) = sum_{k=c}^{nb-1} L21(r, k)
+    // W11(k, c); columns ascending, so S(r, c) overwrites L21(r, c) only
+    // after every column that reads it
+    for (int rr = tid; rr < m2; rr += blockDim.x) {
+      for (int c = 0; c < nb; ++c) {
+        double a = 0.0;
+        for (int k = c; k < nb; ++k) a += Lp[k * ldp + rr] * W11[c * CB + k];
+        Lp[c * ldp + rr] = a;
+      }
+    }
+    __syncthreads();
+    // W21 = -W22 S: row tiles of 16 over rows j1 .. n-1, one wave each
+    const int T = (m2 + 15) >> 4;
+    for (int t = wv; t < T; t += nw) {
+      const int r0 = j1 + 16 * t;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int kend = r0 + 16 < n ? r0 + 16 : n;  // W22(r, l) = 0 for l > r
+      for (int l0 = j1; l0 < kend; l0 += 4) {
+        const int l = l0 + kl, r = r0 + rl;
+        const double a = (l < n && r < n && l <= r) ? -P[pk(r, l, n2)] : 0.0;
+        const double b = (l < n && rl < nb) ? Lp[rl * ldp + (l - j1)] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + kl + 4 * v, c = rl;  // result: row l/16 + 4v, column l%16
+        if (r < n && c < nb) P[pk(r, j0 + c, n2)] = acc[v];
+      }
+    }
+    // W11 into the diagonal block
+    for (int f = tid; f < CB * CB; f += blockDim.x) {
+      const int c = f / CB, r = f - c * CB;
+      if (c < nb && r < nb && r >= c) P[pk(j0 + r, j0 + c, n2)] = W11[c * CB + r];
+    }
+    __syncthreads();
+  }
+  // out = W^T W over lower 16 x 16 tiles
+  const int T = (n + 15) >> 4;
+  const int ntile = T * (T + 1) / 2;
+  for (int t = wv; t < ntile; t += nw) {
+    int ti, tj;
+    lower_tile(t, ti, tj);
+    const int i0 = 16 * ti, c0 = 16 * tj;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = i0; k0 < n; k0 += 4) {  // k >= max(i, j) >= i0
+      const int k = k0 + kl, i = i0 + rl, j = c0 + rl;
+      const double a = (k < n && i < n && k >= i) ? P[pk(k, i, n2)] : 0.0;
+      const double b = (k < n && j < n && k >= j) ? P[pk(k, j, n2)] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + kl + 4 * v, j = c0 + rl;
+      if (i < n && j < n && i >= j) {
+        out[i + (size_t)j * n] = acc[v];
+        out[j + (size_t)i * n] = acc[v];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// C = A B for n x n column-major A, B in device memory, on FP64 MFMA: one
+// wave per 16 x 16 output tile, K in two interleaved chains (independent
+// accumulators keep the MFMA pipeline busy), summed at the end; `out(i, j,
+// v)` stores a result.
+template <class Out>
+__device__ void gemm_mfma64(const double* __restrict__ A, const double* __restrict__ B, int n,
+                            Out out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int rl = lane & 15, kl = lane >> 4;
+  const int T = (n + 15) >> 4;
+  for (int t = wv; t < T * T; t += nw) {
+    const int i0 = 16 * (t % T), j0 = 16 * (t / T);
+    const int i = i0 + rl, j = j0 + rl;
+    f64x4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    for (int k0 = 0; k0 < n; k0 += 8) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int k = k0 + 4 * u + kl;
+        const double a = (i < n && k < n) ? A[i + (size_t)k * n] : 0.0;
+        const double b = (j < n && k < n) ? B[k + (size_t)j * n] : 0.0;
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+      }
+    }
+    const f64x4 s = acc[0] + acc[1];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = i0 + kl + 4 * v, c = j0 + rl;
+      if (r < n && c < n) out(r, c, s[v]);
+    }
+  }
+  __syncthreads();
+}
+
 struct IpmOut {
   double* C;
   int n;
@@ -336,6 +485,12 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       chol_solve1(P, n, n2, b, x);
     else
       chol_solve_lds(P, n, n2, b, x, tA);
+  };
+  auto inverse = [&](double* outm) {
+    if constexpr (P_LDS)
+      potri_packed(P, n, n2, outm, tA);
+    else
+      potri_blocked(P, n, n2, outm, Lp, tB);
   };
   const double* __restrict__ L = nd.L;
   double* X = nd.X;
@@ -378,7 +533,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
     // ---- Zi = Z^{-1}: factor, then invert from the factor
     load_packed(P, n, n2, Z, 0.0, nullptr, nullptr);
     if (chol()) { status = 1; break; }
-    potri_packed(P, n, n2, Zi, tA);
+    inverse(Zi);
     // ---- predictor: dy1 = (Zi .* X) \ (-e)
     for (size_t f = tid; f < nn; f += WG) Mm[f] = Zi[f] * X[f];
     __syncthreads();
@@ -393,7 +548,10 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       T1[f] = -dy1[i] * X[f];
     }
     __syncthreads();
-    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX1, n}, tA, tB);
+    if constexpr (P_LDS)
+      wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX1, n}, tA, tB);
+    else
+      gemm_mfma64(Zi, T1, n, IpmOut{dX1, n});
     __syncthreads();
     for (size_t f = tid; f < nn; f += WG) dX1[f] -= X[f];
     __syncthreads();
@@ -425,7 +583,10 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       T1[f] = -dy[i] * X[f] + (-dy1[i] * dX1[f]);
     }
     __syncthreads();
-    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX, n}, tA, tB);
+    if constexpr (P_LDS)
+      wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX, n}, tA, tB);
+    else
+      gemm_mfma64(Zi, T1, n, IpmOut{dX, n});
     __syncthreads();
     for (size_t f = tid; f < nn; f += WG) dX[f] += mu * Zi[f] + dX1[f];
     for (int i = tid; i < n; i += WG) dy[i] += dy1[i];
