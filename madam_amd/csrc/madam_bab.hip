@@ -540,9 +540,7 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
       dq.push_back(q);
     } else {
       if ((rc = up(sl.L.p, r.PPL.data(), r.PPL.size(), st))) return rc;
-      double fro = 0.0;
-      for (double v : r.PPL) fro += v * v;
-      hv[q].normL = sqrt(fro);
+      hv[q].normL = frob_norm_cols(r.PPL.data(), r.n);
     }
     nmax = std::max(nmax, r.n);
   }
@@ -561,7 +559,12 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     }
   }
   if ((rc = E.views.ensure(k)) || (rc = up(E.views.p, hv.data(), k, st))) return rc;
-  TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, E.lds_limit, st));
+  if (madam::ipm_lds_bytes(nmax, true) <= E.lds_limit) {
+    TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, E.lds_limit, st));
+  } else {
+    TIMED(PH_IPM, st, rc = ipm_tiles(E.views.p, hv.data(), k, nmax, st));
+    if (rc) return rc;
+  }
   HIP_TRY(hipGetLastError());
   if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
   HIP_TRY(hipStreamSynchronize(st));

@@ -33,6 +33,21 @@ inline void set_err(const char* fmt, ...) {
   va_end(ap);
 }
 
+// |L|_F of an n x n column-major matrix in the library's one summation
+// order: each column's squares summed top to bottom, then the n column sums
+// left to right.  k_subproblem sums in the same order (one thread per
+// column), so a node's norm has the same bits whichever side built it.
+inline double frob_norm_cols(const double* L, int n) {
+  double fro = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double* c = L + (size_t)j * n;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += c[i] * c[i];
+    fro += s;
+  }
+  return sqrt(fro);
+}
+
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \
@@ -533,6 +548,11 @@ int upload_spl(int N, const double* SPL);
 int device_subproblems(int N, int k, const int* const* xfixed, const int* const* solX,
                        double* const* dst, int* n_out, double* norm_out);
 void release_spl();
+// The interior-point method for orders whose factor does not fit LDS,
+// spread over many workgroups per node (madam_ipmt.hip); hv: host copies of
+// the views in d_nodes.
+int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream_t st);
+void release_ipm_tiles();
 
 // Resolve the no-allocation path's device array on `device` (current).
 int noalloc_setup(int device);

@@ -218,6 +218,7 @@ void madam_close(void) {
   g_eng->postpart.release();
   g_eng->fviews.release();
   release_spl();
+  release_ipm_tiles();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   delete g_eng;
@@ -267,9 +268,7 @@ int madam_admm_batch(int k, madam_admm_job* jobs, const madam_opts* opts_in) {
     }
     NodeView& v = E.hviews[b];
     bind_view(v, sl, J.n, cs[b]);
-    double fro = 0.0;
-    for (size_t e = 0; e < nn; ++e) fro += J.L[e] * J.L[e];
-    v.normL = sqrt(fro);
+    v.normL = frob_norm_cols(J.L, J.n);
     v.sigma = *J.sigma;
     v.tol = J.tol;
     v.scaleSigma = J.scaleSigma;

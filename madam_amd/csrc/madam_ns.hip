@@ -413,8 +413,20 @@ __global__ __launch_bounds__(64) void k_ns_norm(NodeView* nodes) {
   if (nd.done) return;
   const int n = nd.n;
   const double* __restrict__ Wm = nd.W;
+  const int nn = n * n;
   double fv = 0.0;
-  for (int f = blockIdx.x * 64 + threadIdx.x; f < n * n; f += WG) fv += Wm[f] * Wm[f];
+  int f = blockIdx.x * 64 + threadIdx.x;
+  // the terms of each virtual thread in order, their loads issued 32 at a
+  // time (a latency per 32 terms instead of per term)
+  constexpr int NB = 32;
+  for (; f + (NB - 1) * WG < nn; f += NB * WG) {
+    double v[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) v[t] = Wm[f + t * WG];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) fv += v[t] * v[t];
+  }
+  for (; f < nn; f += WG) fv += Wm[f] * Wm[f];
   fv = wave_sum(fv);
   if (threadIdx.x == 0) ns_norm_parts(nd)[blockIdx.x] = fv;
 }
@@ -524,7 +536,7 @@ void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   auto grid = [&](int parts) { return spread ? dim3(parts, nb) : dim3(8 * parts, ny); };
   const dim3 g32 = grid(ntile);
   k_ns_norm<<<dim3(NS_NORM_WAVES, nb), 64, 0, st>>>(nodes);
-  const int nfill = std::min(64, (np * np + 255) / 256);
+  const int nfill = std::min(256, (np * np + 255) / 256);
   k_ns_fill<<<grid(nfill), 256, 0, st>>>(nodes, nb, spread);
   int sb = 0, tb = 1, ub = 2;
   for (int k = 0; k < MADAM_NS_STEPS; ++k) {
@@ -536,7 +548,7 @@ void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
     sb = tb;
     tb = x;
   }
-  const int nconv = std::min(64, (nmax * nmax + 255) / 256);
+  const int nconv = std::min(256, (nmax * nmax + 255) / 256);
   k_ns_conv<<<grid(nconv), 256, 0, st>>>(nodes, nb, spread, sb);
   const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
   for (int w = 0; w < 3; ++w) k_ns_gemm64<<<grid(ntile64), 256, 0, st>>>(nodes, nb, spread, w);

@@ -102,9 +102,18 @@ __global__ __launch_bounds__(256) void k_subproblem(const double* __restrict__ S
   const size_t nn = (size_t)n * n;
   for (size_t e = tid; e < nn; e += 256) P[e] *= 0.25;
   __syncthreads();
+  // |L|_F in frob_norm_cols's order: column sums (one thread per column,
+  // top to bottom), then their sum left to right
+  for (int j = tid; j < n; j += 256) {
+    const double* c = P + (size_t)j * n;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += c[i] * c[i];
+    frow[j] = s;
+  }
+  __syncthreads();
   if (tid == 0) {
     double fro = 0.0;
-    for (size_t e = 0; e < nn; ++e) fro += P[e] * P[e];
+    for (int j = 0; j < n; ++j) fro += frow[j];
     J.n = n;
     J.normL = sqrt(fro);
   }
