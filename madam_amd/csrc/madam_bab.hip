@@ -20,10 +20,6 @@
 #include "madam_cutlists.h"
 #include "madam_engine.h"
 
-namespace madam {
-void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t st);
-}
-
 using namespace mh;
 using madam::kgon_list;
 using madam::purge_append;
@@ -559,8 +555,8 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     }
   }
   if ((rc = E.views.ensure(k)) || (rc = up(E.views.p, hv.data(), k, st))) return rc;
-  if (madam::ipm_lds_bytes(nmax, true) <= E.lds_limit) {
-    TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, E.lds_limit, st));
+  if (madam::ipm_lds_bytes(nmax) <= E.lds_limit) {
+    TIMED(PH_IPM, st, madam::launch_ipm(E.views.p, k, nmax, st));
   } else {
     TIMED(PH_IPM, st, rc = ipm_tiles(E.views.p, hv.data(), k, nmax, st));
     if (rc) return rc;

@@ -85,8 +85,8 @@ void launch_op_B(const NodeView* nodes, int nb, const double* Xin, double* yout,
 void launch_op_Bt(const NodeView* nodes, int nb, const double* tin, double* Xout, int nmax,
                   hipStream_t stream);
 void launch_project(NodeView* nodes, int nb, int nmax, hipStream_t stream);
-void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t stream);
-size_t ipm_lds_bytes(int n, bool packed);
+void launch_ipm(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+size_t ipm_lds_bytes(int n);
 void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_t lds_limit,
                        hipStream_t stream);
 size_t ns_lds_bytes(int nmax);

@@ -4,7 +4,8 @@
 //
 // Per node: X (nd.X), Z = Diag(y) - L (nd.Q), y (nd.y); scratch Zi (S1),
 // M (S2), dX (W), dX1 (ip1), T1 (ip2), T2 (ip3), vectors in ipv.  Cholesky
-// factors are packed lower triangles, in LDS when they fit, else in ipP.
+// factors are packed lower triangles in LDS: orders whose factor does not
+// fit LDS run in the many-workgroup form of madam_ipmt.hip.
 // Where the reference exits (ipm_mc_pk.c:124-135, 161-165) the kernel
 // stops with ipm_status = 1..3 and the host reports the failure.  Zi is
 // formed as the reference does (dpotrf + dpotri, ipm_mc_pk.c:121-135); the
@@ -165,301 +166,12 @@ __device__ void chol_solve1(const double* P, int n, int n2, const double* b, dou
   __syncthreads();
 }
 
-// Solve (L L^T) x = b with the packed factor in device memory (orders
-// whose factor does not fit LDS), by wave 0 with the vector in LDS (xs, n
-// doubles; b and x may alias): forward sweep column-oriented, b_i -= L_ik
-// x_k below x_k = b_k / L_kk with column k read across the lanes
-// (contiguous); backward sweep x_k = (z_k - sum_{i>k} L_ik x_i) / L_kk with
-// the sum over column k (lanes' partial sums, then a wave reduction).  A
-// wave's LDS accesses complete in order, so no barrier is needed between
-// its steps.
-__device__ void chol_solve_lds(const double* P, int n, int n2, const double* b, double* x,
-                               double* xs) {
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    for (int i = lane; i < n; i += 64) xs[i] = b[i];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (int k = 0; k < n; ++k) {
-      const int cb = pk(k, k, n2) - k;  // column k: P[cb + i] = L(i, k), i >= k
-      const double xk = xs[k] / P[cb + k];
-      for (int i = k + 1 + lane; i < n; i += 64) xs[i] -= P[cb + i] * xk;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) xs[k] = xk;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
-    for (int k = n - 1; k >= 0; --k) {
-      const int cb = pk(k, k, n2) - k;
-      double part = 0.0;
-      for (int i = k + 1 + lane; i < n; i += 64) part += P[cb + i] * xs[i];
-      const double sm = wave_sum(part);
-      const double xk = (xs[k] - sm) / P[cb + k];
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) xs[k] = xk;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
-    for (int i = lane; i < n; i += 64) x[i] = xs[i];
-  }
-  __syncthreads();
-}
-
-// Blocked right-looking Cholesky of the packed lower triangle P (order n,
-// leading order n2) in device memory, for orders whose factor does not fit
-// LDS.  Panels of CB = 16 columns: the panel's rows j0..n-1 are staged in
-// LDS (Lp, CB x ldp doubles, column k at Lp + k ldp), wave 0 factors them
-// (unblocked, one column at a time, no barriers: a wave's LDS accesses are
-// in order), the panel goes back to P, and every wave then applies the
-// panel's rank-CB update to lower 16 x 16 tiles of the trailing block with
-// FP64 MFMA (v_mfma_f64_16x16x4f64: A operand lane l -> (row l % 16, k
-// l / 16), B operand lane l -> (k l / 16, col l % 16), result register v of
-// lane l -> (row l / 16 + 4 v, col l % 16)), three barriers per panel.
-// Returns 0, or c + 1 for the first column c with a non-positive pivot --
-// the same value in every thread.
-constexpr int CB = 16;
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-size_t chol_blocked_lds_doubles(int n) { return (size_t)CB * (((n + 15) & ~15) + 1); }
-
-__device__ int chol_blocked(double* P, int n, int n2, int* flag, double* Lp) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  const int ldp = ((n + 15) & ~15) + 1;  // odd stride: the k groups of a tile read spread banks
-  __syncthreads();  // readers of flag in a previous call are done
-  if (tid == 0) flag[0] = 0;
-  for (int j0 = 0; j0 < n; j0 += CB) {
-    const int nb = n - j0 < CB ? n - j0 : CB, m = n - j0;
-    // stage the panel: Lp[k ldp + (r - j0)] = P(r, j0 + k), r >= j0 + k (else 0)
-    for (int f = tid; f < CB * m; f += blockDim.x) {
-      const int k = f / m, rr = f - k * m;
-      Lp[k * ldp + rr] = (k < nb && rr >= k) ? P[pk(j0 + rr, j0 + k, n2)] : 0.0;
-    }
-    __syncthreads();
-    if (wv == 0) {
-      for (int k = 0; k < nb; ++k) {
-        double* col = Lp + k * ldp;
-        const double d = col[k];
-        if (!(d > 0.0)) {
-          if (lane == 0) flag[0] = j0 + k + 1;
-          break;  // uniform in the wave
-        }
-        const double sd = sqrt(d);
-        for (int rr = k + 1 + lane; rr < m; rr += 64) col[rr] /= sd;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) col[k] = sd;
-        for (int q = k + 1; q < nb; ++q) {
-          double* cq = Lp + q * ldp;
-          const double lq = col[q];
-          for (int rr = q + lane; rr < m; rr += 64) cq[rr] -= col[rr] * lq;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    __syncthreads();
-    if (flag[0]) return flag[0];
-    // the factored panel back to P
-    for (int f = tid; f < CB * m; f += blockDim.x) {
-      const int k = f / m, rr = f - k * m;
-      if (k < nb && rr >= k) P[pk(j0 + rr, j0 + k, n2)] = Lp[k * ldp + rr];
-    }
-    // trailing block rows/columns t0..n-1: lower 16 x 16 tiles, one per wave
-    const int t0 = j0 + nb;
-    if (t0 < n) {
-      const int T = (n - t0 + 15) >> 4;
-      const int ntile = T * (T + 1) / 2;
-      const int rl = lane & 15, kl = lane >> 4;
-      for (int t = wv; t < ntile; t += nw) {
-        int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-        if ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-        if (ti * (ti + 1) / 2 > t) --ti;
-        const int tj = t - ti * (ti + 1) / 2;
-        const int r0 = t0 + 16 * ti, c0 = t0 + 16 * tj;
-        f64x4 acc;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = r0 + kl + 4 * v, c = c0 + rl;
-          acc[v] = (r < n && c < n && r >= c) ? P[pk(r, c, n2)] : 0.0;
-        }
-#pragma unroll
-        for (int kq = 0; kq < CB / 4; ++kq) {
-          const int k = 4 * kq + kl;
-          const int ra = r0 + rl - j0, rb = c0 + rl - j0;  // rows of the panel (< ldp - 1)
-          const double a = (k < nb && r0 + rl < n) ? -Lp[k * ldp + ra] : 0.0;
-          const double b = (k < nb && c0 + rl < n) ? Lp[k * ldp + rb] : 0.0;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = r0 + kl + 4 * v, c = c0 + rl;
-          if (r < n && c < n && r >= c) P[pk(r, c, n2)] = acc[v];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  return 0;
-}
-
-// A 16 x 16 lower tile t of a T x T tile grid: (ti, tj) with ti >= tj.
-__device__ __forceinline__ void lower_tile(int t, int& ti, int& tj) {
-  int a = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-  if ((a + 1) * (a + 2) / 2 <= t) ++a;
-  if (a * (a + 1) / 2 > t) --a;
-  ti = a;
-  tj = t - a * (a + 1) / 2;
-}
-
-// A^{-1} = W^T W, W = L^{-1}, from the packed Cholesky factor L of A in
-// device memory (dpotri for orders beyond LDS).  W in place, blocks of CB
-// columns from the right, with the trailing block W22 = L22^{-1} already
-// in place:  W11 = L11^{-1} (wave 0, in LDS: column c from the right,
-// W11(r, c) = -(1/L_cc) sum_{c<l<=r} W11(r, l) L(l, c), as dtrti2),
-// S = L21 W11 (each thread one row of the panel, in place in LDS), then
-// W21 = -W22 S on FP64 MFMA tiles (16 rows x the block's columns; W22 read
-// from device memory, masked to its lower triangle, S from LDS).  Then
-// out = W^T W (full, leading dimension n) as lower 16 x 16 MFMA tiles,
-// out(i, j) = sum_{k >= max(i, j)} W(k, i) W(k, j), mirrored.
-// LDS: Lp (CB x ldp doubles, the panel) and W11 (CB x CB, in tB).
-__device__ void potri_blocked(double* P, int n, int n2, double* out, double* Lp, double* W11) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  const int ldp = ((n + 15) & ~15) + 1;
-  const int rl = lane & 15, kl = lane >> 4;
-  const int nblk = (n + CB - 1) / CB;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int j0 = bi * CB, nb = n - j0 < CB ? n - j0 : CB, j1 = j0 + nb, m2 = n - j1;
-    // stage L11 (lower, nb x nb) into W11 and L21 (m2 x nb) into the panel
-    for (int f = tid; f < CB * CB; f += blockDim.x) {
-      const int c = f / CB, r = f - c * CB;
-      W11[c * CB + r] = (c < nb && r < nb && r >= c) ? P[pk(j0 + r, j0 + c, n2)] : 0.0;
-    }
-    for (int f = tid; f < CB * m2; f += blockDim.x) {
-      const int k = f / m2, rr = f - k * m2;
-      Lp[k * ldp + rr] = k < nb ? P[pk(j1 + rr, j0 + k, n2)] : 0.0;
-    }
-    __syncthreads();
-    if (wv == 0) {  // W11 = L11^{-1}, columns from the right
-      for (int c = nb - 1; c >= 0; --c) {
-        const double lcc = W11[c * CB + c];
-        double v = 0.0;
-        const int r = c + 1 + lane;  // lanes 0 .. nb-c-2 each own a row below c
-        if (r < nb) {
-          for (int l = c + 1; l <= r; ++l) v += W11[l * CB + r] * W11[c * CB + l];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (r < nb) W11[c * CB + r] = -v / lcc;
-        if (lane == 0) W11[c * CB + c] = 1.0 / lcc;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    __syncthreads();
-    // S = L21 W11, row by row in place: S(r, c) = sum_{k=c}^{nb-1} L21(r, k)
-    // W11(k, c); columns ascending, so S(r, c) overwrites L21(r, c) only
-    // after every column that reads it
-    for (int rr = tid; rr < m2; rr += blockDim.x) {
-      for (int c = 0; c < nb; ++c) {
-        double a = 0.0;
-        for (int k = c; k < nb; ++k) a += Lp[k * ldp + rr] * W11[c * CB + k];
-        Lp[c * ldp + rr] = a;
-      }
-    }
-    __syncthreads();
-    // W21 = -W22 S: row tiles of 16 over rows j1 .. n-1, one wave each
-    const int T = (m2 + 15) >> 4;
-    for (int t = wv; t < T; t += nw) {
-      const int r0 = j1 + 16 * t;
-      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-      const int kend = r0 + 16 < n ? r0 + 16 : n;  // W22(r, l) = 0 for l > r
-      for (int l0 = j1; l0 < kend; l0 += 4) {
-        const int l = l0 + kl, r = r0 + rl;
-        const double a = (l < n && r < n && l <= r) ? -P[pk(r, l, n2)] : 0.0;
-        const double b = (l < n && rl < nb) ? Lp[rl * ldp + (l - j1)] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = r0 + kl + 4 * v, c = rl;  // result: row l/16 + 4v, column l%16
-        if (r < n && c < nb) P[pk(r, j0 + c, n2)] = acc[v];
-      }
-    }
-    // W11 into the diagonal block
-    for (int f = tid; f < CB * CB; f += blockDim.x) {
-      const int c = f / CB, r = f - c * CB;
-      if (c < nb && r < nb && r >= c) P[pk(j0 + r, j0 + c, n2)] = W11[c * CB + r];
-    }
-    __syncthreads();
-  }
-  // out = W^T W over lower 16 x 16 tiles
-  const int T = (n + 15) >> 4;
-  const int ntile = T * (T + 1) / 2;
-  for (int t = wv; t < ntile; t += nw) {
-    int ti, tj;
-    lower_tile(t, ti, tj);
-    const int i0 = 16 * ti, c0 = 16 * tj;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = i0; k0 < n; k0 += 4) {  // k >= max(i, j) >= i0
-      const int k = k0 + kl, i = i0 + rl, j = c0 + rl;
-      const double a = (k < n && i < n && k >= i) ? P[pk(k, i, n2)] : 0.0;
-      const double b = (k < n && j < n && k >= j) ? P[pk(k, j, n2)] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + kl + 4 * v, j = c0 + rl;
-      if (i < n && j < n && i >= j) {
-        out[i + (size_t)j * n] = acc[v];
-        out[j + (size_t)i * n] = acc[v];
-      }
-    }
-  }
-  __syncthreads();
-}
-
-// C = A B for n x n column-major A, B in device memory, on FP64 MFMA: one
-// wave per 16 x 16 output tile, K in two interleaved chains (independent
-// accumulators keep the MFMA pipeline busy), summed at the end; `out(i, j,
-// v)` stores a result.
-template <class Out>
-__device__ void gemm_mfma64(const double* __restrict__ A, const double* __restrict__ B, int n,
-                            Out out) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int rl = lane & 15, kl = lane >> 4;
-  const int T = (n + 15) >> 4;
-  for (int t = wv; t < T * T; t += nw) {
-    const int i0 = 16 * (t % T), j0 = 16 * (t / T);
-    const int i = i0 + rl, j = j0 + rl;
-    f64x4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    for (int k0 = 0; k0 < n; k0 += 8) {
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int k = k0 + 4 * u + kl;
-        const double a = (i < n && k < n) ? A[i + (size_t)k * n] : 0.0;
-        const double b = (j < n && k < n) ? B[k + (size_t)j * n] : 0.0;
-        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
-      }
-    }
-    const f64x4 s = acc[0] + acc[1];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = i0 + kl + 4 * v, c = j0 + rl;
-      if (r < n && c < n) out(r, c, s[v]);
-    }
-  }
-  __syncthreads();
-}
-
 struct IpmOut {
   double* C;
   int n;
   __device__ void operator()(int i, int j, double v) const { C[i + (size_t)j * n] = v; }
 };
 
-template <bool P_LDS>
 __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
@@ -469,29 +181,10 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   double* tA = red + 64;
   double* tB = tA + 16 * 65;
   __shared__ int flag[2];
-  double* P = P_LDS ? (tB + 16 * 65) : nd.ipP;
-  // Cholesky and solves with the packed factor P: in LDS one column at a
-  // time; in device memory (orders too large for LDS) blocked, with the
-  // panel after tB, and the solve's vector in the tA slab
-  double* Lp = tB + 16 * 65;
-  auto chol = [&]() {
-    if constexpr (P_LDS)
-      return chol_packed(P, n, n2, flag);
-    else
-      return chol_blocked(P, n, n2, flag, Lp);
-  };
-  auto solve = [&](const double* b, double* x) {
-    if constexpr (P_LDS)
-      chol_solve1(P, n, n2, b, x);
-    else
-      chol_solve_lds(P, n, n2, b, x, tA);
-  };
-  auto inverse = [&](double* outm) {
-    if constexpr (P_LDS)
-      potri_packed(P, n, n2, outm, tA);
-    else
-      potri_blocked(P, n, n2, outm, Lp, tB);
-  };
+  double* P = tB + 16 * 65;  // the packed Cholesky factor, in LDS
+  auto chol = [&]() { return chol_packed(P, n, n2, flag); };
+  auto solve = [&](const double* b, double* x) { chol_solve1(P, n, n2, b, x); };
+  auto inverse = [&](double* outm) { potri_packed(P, n, n2, outm, tA); };
   const double* __restrict__ L = nd.L;
   double* X = nd.X;
   double* Z = nd.Q;
@@ -548,10 +241,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       T1[f] = -dy1[i] * X[f];
     }
     __syncthreads();
-    if constexpr (P_LDS)
-      wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX1, n}, tA, tB);
-    else
-      gemm_mfma64(Zi, T1, n, IpmOut{dX1, n});
+    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX1, n}, tA, tB);
     __syncthreads();
     for (size_t f = tid; f < nn; f += WG) dX1[f] -= X[f];
     __syncthreads();
@@ -583,10 +273,7 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
       T1[f] = -dy[i] * X[f] + (-dy1[i] * dX1[f]);
     }
     __syncthreads();
-    if constexpr (P_LDS)
-      wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX, n}, tA, tB);
-    else
-      gemm_mfma64(Zi, T1, n, IpmOut{dX, n});
+    wg_gemm<false, false>(Zi, n, T1, n, nullptr, n, n, n, IpmOut{dX, n}, tA, tB);
     __syncthreads();
     for (size_t f = tid; f < nn; f += WG) dX[f] += mu * Zi[f] + dX1[f];
     for (int i = tid; i < n; i += WG) dy[i] += dy1[i];
@@ -646,17 +333,14 @@ __global__ __launch_bounds__(WG) void k_ipm(NodeView* nodes, int maxit) {
   }
 }
 
-size_t ipm_lds_bytes(int n, bool packed) {
-  size_t d = 64 + 2 * 16 * 65;
-  d += packed ? (size_t)n * (n + 1) / 2 : chol_blocked_lds_doubles(n);
-  return d * sizeof(double);
+size_t ipm_lds_bytes(int n) {
+  return (64 + 2 * 16 * 65 + (size_t)n * (n + 1) / 2) * sizeof(double);
 }
 
-void launch_ipm(NodeView* nodes, int nb, int nmax, size_t lds_limit, hipStream_t st) {
-  if (ipm_lds_bytes(nmax, true) <= lds_limit)
-    k_ipm<true><<<nb, WG, ipm_lds_bytes(nmax, true), st>>>(nodes, 500);
-  else
-    k_ipm<false><<<nb, WG, ipm_lds_bytes(nmax, false), st>>>(nodes, 500);
+// One workgroup per node; orders whose packed factor does not fit LDS run
+// in the many-workgroup form of madam_ipmt.hip instead.
+void launch_ipm(NodeView* nodes, int nb, int nmax, hipStream_t st) {
+  k_ipm<<<nb, WG, ipm_lds_bytes(nmax), st>>>(nodes, 500);
 }
 
 }  // namespace madam
