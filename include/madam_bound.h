@@ -68,8 +68,10 @@ typedef struct {
     int check_every;       /* ADMM iterations between host checks of the stop flags, default 16 */
     int projection;        /* 0: FP64 eigen-decomposition (Jacobi); 1: mixed-precision sign
                               iteration (FP32 MFMA quintic Newton-Schulz + FP64 finish) */
-    int ns_layout;         /* sign iteration: 0 automatic, 1 one workgroup per node,
-                              2 tile-parallel (each product spread over many workgroups) */
+    int ns_layout;         /* sign iteration: 0 automatic (the madam_load_tuning table, else
+                              built-in crossovers), 1 one workgroup per node, 2 tile-parallel
+                              (each product spread over many workgroups), 3 tile-parallel
+                              over 2 x 2 blocks of tiles */
     double cg_adapt;       /* > 0: after an ADMM call's first iteration, PCG stops at the
                               relative residual max(cg_rtol, min(1e-3, cg_adapt * the
                               previous iteration's ADMM error)); 0: cg_rtol throughout */
@@ -148,6 +150,23 @@ int madam_project_sdp(int n, const double *W, double *X, double *Q, double *Z, i
  * with sign(W) from the FP32-MFMA quintic sign iteration and one FP64
  * correction step, Q = X - W, M = number of positive eigenvalues. */
 int madam_project_sdp_ns(int n, const double *W, double *X, double *Q, int *M);
+
+/* Autotuning of the sign-iteration projection's launch layout (no
+ * reference counterpart: the reference has one dsyevr call per projection,
+ * operators.c:579).  Layouts: 1 one workgroup per node, 2 one launch per
+ * product over 16 x 16 tiles, 3 the same over 2 x 2 blocks of tiles; all
+ * give the same bits.
+ * madam_time_projection: average milliseconds of one projection of nb
+ * synthetic nodes of order n in `layout` over reps calls.
+ * madam_load_tuning: load the table (text lines "n_p nodes layout ...",
+ * '#' comments) that picks the layout from the order rounded up to 16 and
+ * the number of active nodes; null or "" clears it (the built-in
+ * crossovers again).  Host only.
+ * madam_tuned_layout: the loaded table's layout for order n and `active`
+ * nodes (0 without a table). */
+int madam_time_projection(int n, int nb, int layout, int reps, double *ms);
+int madam_load_tuning(const char *path);
+int madam_tuned_layout(int n, int active);
 
 /* Coefficients (a_k, b_k, c_k) of the sign iteration's steps
  * x -> a x + b x^3 + c x^5 used by the mixed-precision projection (no

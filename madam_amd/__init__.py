@@ -19,6 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libmadam_bound.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "madam_bound.h")
+# sign-iteration layout table written by madam_amd/tune.py on an MI355X
+TUNE_TABLE = os.path.join(HERE, "ns_tune.tsv")
 
 _PD = C.POINTER(C.c_double)
 _PI = C.POINTER(C.c_int)
@@ -85,6 +87,8 @@ def lib():
         L.madam_last_error.restype = C.c_char_p
         L.madam_admm_batch.argtypes = [C.c_int, C.POINTER(AdmmJob), C.POINTER(Opts)]
         _lib = L
+        if os.path.exists(TUNE_TABLE):  # the autotuner's layout table (tune.py)
+            _check(L.madam_load_tuning(TUNE_TABLE.encode()))
     return _lib
 
 
@@ -95,6 +99,26 @@ def _check(rc):
 
 def open_device(device: int = 0):
     _check(lib().madam_open(C.c_int(device)))
+
+
+def time_projection(n, nb, layout, reps=3):
+    """Average milliseconds of one sign-iteration projection of nb synthetic
+    nodes of order n in `layout` (1 one workgroup per node, 2 tiles, 3 2 x 2
+    tiles)."""
+    ms = C.c_double()
+    _check(lib().madam_time_projection(C.c_int(n), C.c_int(nb), C.c_int(layout), C.c_int(reps),
+                                       C.byref(ms)))
+    return ms.value
+
+
+def load_tuning(path=None):
+    """Load a layout table (None: clear it; the built-in crossovers)."""
+    _check(lib().madam_load_tuning(None if path is None else str(path).encode()))
+
+
+def tuned_layout(n, active):
+    """The loaded table's layout for order n and `active` nodes (0: none)."""
+    return int(lib().madam_tuned_layout(C.c_int(n), C.c_int(active)))
 
 
 def _f64(a):
@@ -237,7 +261,8 @@ def _make_job(L, X, y, Q, tri, pent, hept, sigma, tol, itermax, scaleSigma, keep
 def _opts(cg_rtol=1e-12, cg_maxit=1000, jacobi_sweeps=40, check_every=16, projection=0,
           ns_layout=0, cg_adapt=0.0):
     """projection: 0 FP64 eigen (Jacobi), 1 mixed-precision sign iteration;
-    ns_layout: 0 automatic, 1 one workgroup per node, 2 tile-parallel;
+    ns_layout: 0 automatic (the loaded tuning table, else built-in crossovers),
+    1 one workgroup per node, 2 tile-parallel, 3 tile-parallel over 2 x 2 tiles;
     cg_adapt: > 0 loosens the PCG tolerance to cg_adapt x the ADMM error."""
     o = Opts()
     o.cg_rtol, o.cg_maxit, o.jacobi_sweeps, o.check_every = cg_rtol, cg_maxit, jacobi_sweeps, check_every

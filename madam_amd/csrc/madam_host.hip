@@ -32,6 +32,43 @@ Engine* g_eng = nullptr;
 }  // namespace mh
 using namespace mh;
 
+// Sign-iteration layout table written by the autotuner (madam_amd/tune.py,
+// loaded by madam_load_tuning): cells (n_p, active nodes) -> layout (1 one
+// workgroup per node, 2 tiles, 3 2 x 2 tiles).  Empty: the built-in
+// crossovers of admm_advance.
+namespace {
+struct TuneCell {
+  int np, nb, layout;
+};
+std::vector<TuneCell> g_tune;
+
+// The cell of the smallest tabulated n_p >= np (else the largest) and of
+// the largest tabulated node count <= active (else the smallest); 0
+// without a table.
+int tuned_layout(int np, int active) {
+  if (g_tune.empty()) return 0;
+  int sel = -1, top = -1;
+  for (const TuneCell& c : g_tune) {
+    if (c.np >= np && (sel < 0 || c.np < sel)) sel = c.np;
+    top = std::max(top, c.np);
+  }
+  if (sel < 0) sel = top;
+  int best_nb = -1, lay = 0, min_nb = 0, lay_min = 0;
+  for (const TuneCell& c : g_tune) {
+    if (c.np != sel) continue;
+    if (c.nb <= active && c.nb > best_nb) {
+      best_nb = c.nb;
+      lay = c.layout;
+    }
+    if (lay_min == 0 || c.nb < min_nb) {
+      min_nb = c.nb;
+      lay_min = c.layout;
+    }
+  }
+  return best_nb >= 0 ? lay : lay_min;
+}
+}  // namespace
+
 int mh::admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax) {
   int rc = admm_advance(hv, o, nmax, mmax, 0);
   if (rc) return rc;
@@ -73,13 +110,14 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
     // 96 active nodes at n_p = 112 (0.13 ms for one node), 16 at n_p = 80,
     // never at 64; larger orders: tiles below ~3/4 of the CU count (n = 150:
     // 1.0 against 1.13 ms at 192 nodes, 1.31 against 1.20 ms at 256)
+    // (when no autotuned table is loaded)
     const bool small = madam::ns_lds_bytes(nmax) + 1024 <= E.lds_limit;
     const int np = (nmax + 15) & ~15;
     const int small_tiles = np >= 112 ? 96 : (np >= 96 ? 48 : (np >= 80 ? 16 : 0));
-    const bool tiles =
-        o.ns_layout == 2 ||
-        (o.ns_layout == 0 &&
-         (small ? active <= small_tiles : active < (3 * E.cu_count) / 4));
+    int layout = o.ns_layout;
+    if (layout == 0) layout = tuned_layout(np, active);
+    if (layout == 0)
+      layout = (small ? active <= small_tiles : active < (3 * E.cu_count) / 4) ? 2 : 1;
     for (int it = 0; it < chunk; ++it) {
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
@@ -88,10 +126,10 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
         TIMED(PH_EIG, st, {
-          if (tiles)
-            madam::launch_ns_tiles(E.views.p, k, nmax, st);
-          else
+          if (layout == 1)
             madam::launch_ns_project(E.views.p, k, nmax, 0, E.lds_limit, st);
+          else
+            madam::launch_ns_tiles(E.views.p, k, nmax, st, layout == 3);
           madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 3);
         });
       } else {
@@ -675,6 +713,93 @@ int madam_project_sdp_ns(int n, const double* W, double* X, double* Q, int* M) {
   HIP_TRY(hipStreamSynchronize(st));
   g_eng->timer.flush();
   if (M) *M = g_eng->hviews[0].M;
+  return MADAM_OK;
+}
+
+int madam_load_tuning(const char* path) {
+  std::vector<TuneCell> cells;
+  if (path && *path) {
+    FILE* f = fopen(path, "r");
+    if (!f) {
+      set_err("madam_load_tuning: cannot open %s", path);
+      return MADAM_E_ARG;
+    }
+    char line[512];
+    int ln = 0;
+    while (fgets(line, sizeof line, f)) {
+      ++ln;
+      const char* p = line;
+      while (*p == ' ' || *p == '\t') ++p;
+      if (*p == '#' || *p == '\n' || *p == '\r' || *p == '\0') continue;
+      TuneCell c;
+      if (sscanf(p, "%d %d %d", &c.np, &c.nb, &c.layout) != 3 || c.np < 16 || c.np % 16 != 0 ||
+          c.np > ((MADAM_NMAX + 15) & ~15) || c.nb < 1 || c.layout < 1 || c.layout > 3) {
+        fclose(f);
+        set_err("madam_load_tuning: %s line %d: expected 'n_p nodes layout' (n_p a multiple of "
+                "16 up to %d, nodes >= 1, layout 1..3)", path, ln, (MADAM_NMAX + 15) & ~15);
+        return MADAM_E_ARG;
+      }
+      cells.push_back(c);
+    }
+    fclose(f);
+    if (cells.empty()) {
+      set_err("madam_load_tuning: %s holds no cells", path);
+      return MADAM_E_ARG;
+    }
+  }
+  g_tune.swap(cells);
+  return MADAM_OK;
+}
+
+int madam_tuned_layout(int n, int active) { return tuned_layout((n + 15) & ~15, active); }
+
+int madam_time_projection(int n, int nb, int layout, int reps, double* ms) {
+  int rc = ensure_open();
+  if (rc) return rc;
+  if (n < 2 || n > MADAM_NMAX || nb < 1 || nb > 1024 || layout < 1 || layout > 3 || reps < 1 ||
+      !ms) {
+    set_err("madam_time_projection: bad arguments (n=%d nb=%d layout=%d reps=%d)", n, nb, layout,
+            reps);
+    return MADAM_E_ARG;
+  }
+  if ((rc = ensure_slots(nb))) return rc;
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  CutSet cs;
+  if ((rc = build_cutset(n, 0, nullptr, 0, nullptr, 0, nullptr, cs))) return rc;
+  for (int b = 0; b < nb; ++b) {
+    Slot& sl = E.slots[b];
+    if ((rc = sl.size_for(n, cs)) || (rc = sl.size_ns(n))) return rc;
+    bind_view(E.hviews[b], sl, n, cs);
+  }
+  if ((rc = up(E.views.p, E.hviews.data(), nb, st))) return rc;
+  madam::launch_ns_synth(E.views.p, nb, n, st);
+  auto run = [&]() {
+    if (layout == 1)
+      madam::launch_ns_project(E.views.p, nb, n, 0, E.lds_limit, st);
+    else
+      madam::launch_ns_tiles(E.views.p, nb, n, st, layout == 3);
+  };
+  run();  // warm-up
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  struct Events {  // back to the timer's pool on every return
+    PhaseTimer& t;
+    hipEvent_t e[2];
+    ~Events() {
+      t.pool.push_back(e[0]);
+      t.pool.push_back(e[1]);
+    }
+  } evs{E.timer, {E.timer.get(), E.timer.get()}};
+  hipEvent_t* ev = evs.e;
+  HIP_TRY(hipEventRecord(ev[0], st));
+  for (int r = 0; r < reps; ++r) run();
+  HIP_TRY(hipEventRecord(ev[1], st));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(ev[1]));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+  *ms = (double)t / reps;
   return MADAM_OK;
 }
 

@@ -90,7 +90,9 @@ size_t ipm_lds_bytes(int n);
 void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_t lds_limit,
                        hipStream_t stream);
 size_t ns_lds_bytes(int nmax);
-void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t stream);
+// blocks2: the 2 x 2-tile product kernel (same bits)
+void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t stream, int blocks2);
+void launch_ns_synth(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_zfac_chol(NodeView* nodes, int nb, hipStream_t stream);
 void launch_safe_hhg(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 size_t safe_hh_lds_bytes(int nmax);

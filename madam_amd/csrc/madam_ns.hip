@@ -480,6 +480,97 @@ __global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int 
                    ((acc + part[0][lane]) + part[1][lane]) + part[2][lane]);
 }
 
+// The same product on 2 x 2 blocks of tiles: workgroup t covers lower
+// block t of the ceil(n_p / 32)-square block grid with two waves, wave w
+// accumulating K quarters 2w and 2w + 1 of all four tiles, so that every
+// operand value loaded feeds two MFMAs (half the loads per flop of
+// k_ns_gemm32).  Each tile's quarters run the same k steps in the same
+// order and are summed as ((q0 + q1) + q2) + q3: the same bits.  Faster
+// for batches of large orders (n = 512, 32 nodes: 78 against 114 us per
+// product), slower for one node (fewer, longer waves).
+__global__ __launch_bounds__(128) void k_ns_gemm32b(NodeView* nodes, int nb, int spread, int ia,
+                                                    int ib, int ic, float alpha, float beta) {
+  constexpr int NSB = 8;
+  __shared__ f32x4 red[2][4][64];
+  int t, nt_unused;
+  const int b = ns_node(nb, spread, t, nt_unused);
+  if (b < 0) return;
+  NodeView& nd = nodes[b];
+  if (nd.done) return;
+  const int np = (nd.n + 15) & ~15, nt = np >> 4, ng = (nt + 1) >> 1;
+  if (t >= ng * (ng + 1) / 2) return;
+  int bi, bj;
+  tile_decode(t, bi, bj);
+  bi >>= 4;
+  bj >>= 4;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const size_t npp = (size_t)np * np;
+  const float* __restrict__ A = nd.ns32 + ia * npp;
+  const float* __restrict__ B = nd.ns32 + ib * npp;
+  bool vr[2], vc[2];
+  int ra[2], cb[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    vr[u] = 2 * bi + u < nt;
+    vc[u] = 2 * bj + u < nt;
+    ra[u] = (vr[u] ? (2 * bi + u) * 16 : 0) + r;  // a missing tile row reads row 0, unused
+    cb[u] = (vc[u] ? (2 * bj + u) * 16 : 0) + r;
+  }
+  bool on[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    on[x] = vr[x >> 1] && vc[x & 1] && 2 * bj + (x & 1) <= 2 * bi + (x >> 1);
+  // quarter c = k steps [c nt, (c + 1) nt) of 4 k each (lane group q holds k = 4s + q)
+  f32x4 acc[2][4];
+#pragma unroll
+  for (int e = 0; e < 2; ++e)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) acc[e][x] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int s0 = 0; s0 < nt; s0 += NSB) {
+    float a[NSB][2][2], bq[NSB][2][2];
+#pragma unroll
+    for (int s = 0; s < NSB; ++s)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int st = s0 + s < nt ? s0 + s : 0;  // past the quarter: in bounds, unused
+        const int k = 4 * ((2 * w + e) * nt + st) + q;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          a[s][e][u] = A[ra[u] + (size_t)k * np];
+          float bb = beta * B[cb[u] + (size_t)k * np];
+          if (cb[u] == k) bb += alpha;
+          bq[s][e][u] = bb;
+        }
+      }
+#pragma unroll
+    for (int s = 0; s < NSB; ++s) {
+      if (s0 + s >= nt) break;
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          if (on[x])
+            acc[e][x] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s][e][x >> 1], bq[s][e][x & 1],
+                                                             acc[e][x], 0, 0, 0);
+    }
+  }
+  if (w == 1) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int x = 0; x < 4; ++x) red[e][x][lane] = acc[e][x];
+  }
+  __syncthreads();
+  if (w != 0) return;
+  float* Cm = nd.ns32 + ic * npp;
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    if (on[x])
+      store_tile_f32(Cm, np, (2 * bi + (x >> 1)) * 16, (2 * bj + (x & 1)) * 16,
+                     ((acc[0][x] + acc[1][x]) + red[0][x][lane]) + red[1][x][lane]);
+}
+
 // FP32 iterate (buffer is) -> FP64 n x n in S2, with the guard.
 __global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int spread, int is) {
   int part, nparts;
@@ -525,25 +616,32 @@ __global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int 
   if (w == 0) store_tile_f64(Cm, n, i0, j0, ((acc + part[0][lane]) + part[1][lane]) + part[2][lane]);
 }
 
-void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
+void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st, int blocks2) {
   static const float coef[MADAM_NS_STEPS][3] = MADAM_NS_COEF_INIT;
   const int np = (nmax + 15) & ~15, nt = np >> 4, ntile = nt * (nt + 1) / 2;
+  const int ng = (nt + 1) >> 1, nblk = ng * (ng + 1) / 2;
   // spread a node over all XCDs only when its tiles outnumber one XCD's 32
   // CUs (n = 512: 528 tiles); below that one L2 serving every tile is
   // faster (n = 100, 28 tiles: 0.17 ms per projection spread, 0.135 local)
   const int spread = (nb < 8 && ntile > 48) ? 1 : 0;
   const int ny = (nb + 7) / 8;
   auto grid = [&](int parts) { return spread ? dim3(parts, nb) : dim3(8 * parts, ny); };
-  const dim3 g32 = grid(ntile);
+  const dim3 g32 = grid(blocks2 ? nblk : ntile);
+  const int wg32 = blocks2 ? 128 : 256;
+  auto gemm = [&](int ia, int ib, int ic, float alpha, float beta) {
+    if (blocks2)
+      k_ns_gemm32b<<<g32, wg32, 0, st>>>(nodes, nb, spread, ia, ib, ic, alpha, beta);
+    else
+      k_ns_gemm32<<<g32, wg32, 0, st>>>(nodes, nb, spread, ia, ib, ic, alpha, beta);
+  };
   k_ns_norm<<<dim3(NS_NORM_WAVES, nb), 64, 0, st>>>(nodes);
   const int nfill = std::min(256, (np * np + 255) / 256);
   k_ns_fill<<<grid(nfill), 256, 0, st>>>(nodes, nb, spread);
   int sb = 0, tb = 1, ub = 2;
   for (int k = 0; k < MADAM_NS_STEPS; ++k) {
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, sb, sb, tb, 0.f, 1.f);  // T = S^2
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, tb, tb, ub, coef[k][1],
-                                     coef[k][2]);                                // U
-    k_ns_gemm32<<<g32, 256, 0, st>>>(nodes, nb, spread, sb, ub, tb, coef[k][0], 1.f);  // S'
+    gemm(sb, sb, tb, 0.f, 1.f);                   // T = S^2
+    gemm(tb, tb, ub, coef[k][1], coef[k][2]);     // U = T (b I + c T)
+    gemm(sb, ub, tb, coef[k][0], 1.f);            // S' = S (a I + U)
     const int x = sb;
     sb = tb;
     tb = x;
@@ -552,6 +650,29 @@ void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t st) {
   k_ns_conv<<<grid(nconv), 256, 0, st>>>(nodes, nb, spread, sb);
   const int nt64 = (nmax + 15) >> 4, ntile64 = nt64 * (nt64 + 1) / 2;
   for (int w = 0; w < 3; ++w) k_ns_gemm64<<<grid(ntile64), 256, 0, st>>>(nodes, nb, spread, w);
+}
+
+// A synthetic symmetric W for timing the projection layouts: entry (i, j)
+// a hash of the unordered pair and the node, uniform in [-1, 1).
+__global__ __launch_bounds__(256) void k_ns_synth(NodeView* nodes) {
+  NodeView& nd = nodes[blockIdx.y];
+  const int n = nd.n;
+  const size_t nn = (size_t)n * n;
+  for (size_t f = blockIdx.x * 256 + threadIdx.x; f < nn; f += (size_t)gridDim.x * 256) {
+    const int j = (int)(f / n), i = (int)(f - (size_t)j * n);
+    const unsigned long long lo = i < j ? i : j, hi = i < j ? j : i;
+    unsigned long long h = lo * 0x9E3779B97F4A7C15ull + hi * 0xC2B2AE3D27D4EB4Full + blockIdx.y;
+    h ^= h >> 30;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 27;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    nd.W[f] = (double)(h >> 11) * 0x1.0p-52 - 1.0;
+  }
+}
+
+void launch_ns_synth(NodeView* nodes, int nb, int nmax, hipStream_t st) {
+  k_ns_synth<<<dim3(std::min(256, (nmax * nmax + 255) / 256), nb), 256, 0, st>>>(nodes);
 }
 
 // ---------------------------------------------------------------------------

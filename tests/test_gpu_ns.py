@@ -14,6 +14,8 @@
                      within 1e-4 relative of the FP64 oracle (measured
                      1.1e-5 on g05_60.0, the sign-iteration bound tighter)
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -122,17 +124,20 @@ def _layout_graph(spec):
                                         ("gnp120", False), ("gnp140", False),
                                         ("gnp150", False), ("gnp200", False)])
 def test_tile_parallel_layout_is_bit_identical(graph, kgon):
-    """The tile-parallel launch sequence computes the same tiles with the
-    same instruction order as the one-workgroup-per-node kernel in each of
-    its storage modes: the ADMM results must agree bit for bit."""
+    """The tile-parallel launch sequences (16 x 16 tiles, and 2 x 2 blocks of
+    tiles) compute the same tiles with the same instruction order as the
+    one-workgroup-per-node kernel in each of its storage modes: the ADMM
+    results must agree bit for bit."""
     sol, case = admm_case(graph_L(_layout_graph(graph)), kgon=kgon)
     case["tol"] = 1e-3
     a = G.ADMM_solver(**case, projection=1, ns_layout=1)
     b = G.ADMM_solver(**case, projection=1, ns_layout=2)
-    print(graph, "layout1", a.f, a.nbit, "layout2", b.f, b.nbit)
-    assert a.nbit == b.nbit
-    assert a.f == b.f
-    assert np.array_equal(a.X, b.X)
+    c = G.ADMM_solver(**case, projection=1, ns_layout=3)
+    print(graph, "layout1", a.f, a.nbit, "layout2", b.f, b.nbit, "layout3", c.f, c.nbit)
+    for o in (b, c):
+        assert a.nbit == o.nbit
+        assert a.f == o.f
+        assert np.array_equal(a.X, o.X)
 
 
 def test_batch_of_mixed_orders_is_bit_identical():
@@ -169,3 +174,20 @@ def test_heuristic_factor_reproduces_X():
           "max |X|", np.abs(X).max())
     assert Z.shape[1] >= 1
     assert err <= 20 * neg + 1e-8 * np.abs(X).max()
+
+
+def test_layout_timing_and_tuner(tmp_path):
+    """madam_time_projection times each layout, and the autotuner writes a
+    table that the library loads and looks up."""
+    from madam_amd import tune
+    for lay in (1, 2, 3):
+        assert G.time_projection(100, 2, lay, 1) > 0.0
+    out = str(tmp_path / "t.tsv")
+    rows = tune.tune(out, nps=[64, 160], nbs=[1, 4], reps=1)
+    assert len(rows) == 4
+    try:
+        G.load_tuning(out)
+        for np_, nb, best, ms in rows:
+            assert G.tuned_layout(np_, nb) == best
+    finally:
+        G.load_tuning(G.TUNE_TABLE if os.path.exists(G.TUNE_TABLE) else None)
