@@ -289,6 +289,63 @@ def wl_bab150(args):
 C5_PARAMS = dict(TriIneq=3000, adjust_TriIneq=0, PentIneq=1000, HeptaIneq=1000)
 
 
+def vendor_fp64_eig(n, reps=5):
+    """AMD's own FP64 symmetric eigensolver on this GPU as a reference point
+    for the projection (measurement only, nothing of the product uses it):
+    rocSOLVER dsyevd -- with eigenvectors (what an FP64 project_SDP needs,
+    operators.c:579 calls dsyevr) and values only -- on a random symmetric
+    matrix of order n, ms per call from HIP events around the call."""
+    import madam_amd as G
+    G.lib()  # the HIP runtime the library loaded
+    hip = C.CDLL("libamdhip64.so.7", mode=getattr(os, "RTLD_NOLOAD", 4) | C.RTLD_GLOBAL)
+    rb = C.CDLL("/opt/rocm/lib/librocblas.so")
+    rs = C.CDLL("/opt/rocm/lib/librocsolver.so")
+    V = C.c_void_p
+
+    def ck(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: error {rc}")
+
+    h = V()
+    ck(rb.rocblas_create_handle(C.byref(h)), "rocblas_create_handle")
+    bufs = [V() for _ in range(5)]
+    sizes = [8 * n * n, 8 * n * n, 8 * n, 8 * n, 4]
+    try:
+        for b, sz in zip(bufs, sizes):
+            ck(hip.hipMalloc(C.byref(b), C.c_size_t(sz)), "hipMalloc")
+        A, A0, D, E, info = bufs
+        rng = np.random.default_rng(n)
+        M = rng.uniform(-0.5, 0.5, (n, n))
+        M = np.asfortranarray(np.tril(M) + np.tril(M, -1).T)
+        ck(hip.hipMemcpy(A0, M.ctypes.data_as(V), C.c_size_t(8 * n * n), 1), "hipMemcpy")
+        ev = [V(), V()]
+        for e in ev:
+            ck(hip.hipEventCreate(C.byref(e)), "hipEventCreate")
+        out = {}
+        for name, evect in (("ms_with_vectors", 211), ("ms_values_only", 213)):
+            tot = 0.0
+            for r in range(reps + 1):
+                ck(hip.hipMemcpy(A, A0, C.c_size_t(8 * n * n), 3), "hipMemcpy")
+                ck(hip.hipEventRecord(ev[0], V()), "hipEventRecord")
+                ck(rs.rocsolver_dsyevd(h, C.c_int(evect), C.c_int(122), C.c_int(n), A, C.c_int(n),
+                                       D, E, info), "rocsolver_dsyevd")
+                ck(hip.hipEventRecord(ev[1], V()), "hipEventRecord")
+                ck(hip.hipEventSynchronize(ev[1]), "hipEventSynchronize")
+                ms = C.c_float()
+                ck(hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1]), "hipEventElapsedTime")
+                if r > 0:  # the first call warms up
+                    tot += ms.value
+            out[name] = tot / reps
+        for e in ev:
+            hip.hipEventDestroy(e)
+        return {"library": "rocSOLVER dsyevd (FP64)", "n": n, **out}
+    finally:
+        for b in bufs:
+            if b.value:
+                hip.hipFree(b)
+        rb.rocblas_destroy_handle(h)
+
+
 def _cpu_info():
     model = "unknown"
     try:
@@ -479,6 +536,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-replay", action="store_true", help="c5: skip the identical-inputs replay")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the rocSOLVER dsyevd reference timing of one projection")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-workers", type=int, default=0,
                     help="c5: oracle worker processes (0: the cores this process may use, "
@@ -493,7 +552,7 @@ def main():
                     help="c5: node bounds per step (root included)")
     ap.add_argument("--ns-n", type=int, default=150, help="ns-batch: node order")
     ap.add_argument("--ns-k", type=int, default=128, help="ns-batch: nodes in the batch")
-    ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2, 3],
                     help="chosen form's sign-iteration layout: 0 automatic, 1 one workgroup "
                          "per node, 2 tile-parallel")
     ap.add_argument("--timed-form", default="chosen", choices=["chosen", "plain"],
@@ -648,6 +707,15 @@ def main():
     if args.workload == "c1-bab" or (args.workload == "bab150" and r.get("open") == 0):
         line["time_to_optimum_s"] = ms_per_step / 1e3
         line["optimum"] = r["bound"]
+    if not args.no_vendor and args.workload in ("c5", "c2-root") and roofline is not None:
+        try:
+            ve = vendor_fp64_eig(int(rp["n"]))
+            ve["chosen_ms_per_projection"] = roofline["ms_per_projection"]
+            ve["chosen_speedup_vs_vectors"] = ve["ms_with_vectors"] / roofline["ms_per_projection"]
+            line["vendor_fp64_eigensolver"] = ve
+            log("vendor FP64 eigensolver reference done")
+        except (OSError, RuntimeError, AttributeError) as e:
+            line["vendor_fp64_eigensolver"] = {"error": str(e)}
     if args.workload == "c4-learned" and plain is not None:
         line["admm_iterations"] = {"warm": r["admm_iters"], "cold": plain["admm_iters"],
                                    "saved_frac": 1.0 - r["admm_iters"] / plain["admm_iters"]}

@@ -179,9 +179,10 @@ int madam_ns_schedule(int cap, float *coef);
 int madam_lambda_max(int n, const double *A, double *lam);
 
 /* The same bound as the mixed-precision form computes it: an estimate
- * (Householder tridiagonalisation and Sturm multisection, in LDS for n up
- * to about 195, in device memory beyond) certified by an FP64 Cholesky
- * factorisation of mu I - A (lam >= lambda_max(A), within ~1e-10 |A|_F). */
+ * (Householder tridiagonalisation and Sturm multisection: one workgroup
+ * with A in LDS for n up to about 195, many workgroups with A in device
+ * memory beyond) certified by an FP64 Cholesky factorisation of mu I - A
+ * (lam >= lambda_max(A), within ~1e-10 |A|_F). */
 int madam_lambda_max_cert(int n, const double *A, double *lam);
 
 /* getViolated_TriangleInequalities (cutting_planes.c:48-127): the at most

@@ -553,6 +553,11 @@ void release_spl();
 // the views in d_nodes.
 int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream_t st);
 void release_ipm_tiles();
+// The safe bound's certified lambda_max for orders beyond LDS, spread over
+// many workgroups per node (madam_cert.hip): A in each node's W, the value
+// in ev (what k_safe_bound reads).
+int safe_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream_t st);
+void release_safe_tiles();
 
 // Resolve the no-allocation path's device array on `device` (current).
 int noalloc_setup(int device);

@@ -166,17 +166,19 @@ int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, in
   TIMED(PH_SAFE, st, {
     madam::launch_build_w(E.fviews.p, k, nmax, mmax, 1, st);
     // chosen form: an estimate of lambda_max proved an upper bound by a
-    // Cholesky certificate -- the estimate by Householder tridiagonalisation
-    // in LDS, or by Lanczos for nodes too large for that; plain form: all
+    // Cholesky certificate -- Householder tridiagonalisation and the
+    // Cholesky in LDS by one workgroup per node, or spread over many
+    // workgroups for orders too large for that; plain form: all
     // eigenvalues by Jacobi, as dsyevr in the reference
     if (o.projection == 1 && madam::safe_hh_lds_bytes(nmax) + 1024 <= E.lds_limit)
       madam::launch_safe_hh(E.fviews.p, k, nmax, st);
     else if (o.projection == 1)
-      madam::launch_safe_hhg(E.fviews.p, k, nmax, st);
+      rc = safe_tiles(E.fviews.p, hv.data(), k, nmax, st);
     else
       madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
-    madam::launch_safe_bound(E.fviews.p, k, st);
+    if (!rc) madam::launch_safe_bound(E.fviews.p, k, st);
   });
+  if (rc) return rc;
   HIP_TRY(hipGetLastError());
   if ((rc = down(hv.data(), E.fviews.p, k, st))) return rc;
   HIP_TRY(hipStreamSynchronize(st));
@@ -257,6 +259,7 @@ void madam_close(void) {
   g_eng->fviews.release();
   release_spl();
   release_ipm_tiles();
+  release_safe_tiles();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   delete g_eng;
@@ -457,8 +460,8 @@ int madam_lambda_max_cert(int n, const double* A, double* lam) {
   HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
   if (madam::safe_hh_lds_bytes(n) + 1024 <= g_eng->lds_limit)
     madam::launch_safe_hh(g_eng->views.p, 1, n, st);
-  else
-    madam::launch_safe_hhg(g_eng->views.p, 1, n, st);
+  else if ((rc = safe_tiles(g_eng->views.p, g_eng->hviews.data(), 1, n, st)))
+    return rc;
   madam::launch_safe_bound(g_eng->views.p, 1, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;

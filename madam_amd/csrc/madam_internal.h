@@ -94,7 +94,6 @@ size_t ns_lds_bytes(int nmax);
 void launch_ns_tiles(NodeView* nodes, int nb, int nmax, hipStream_t stream, int blocks2);
 void launch_ns_synth(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_zfac_chol(NodeView* nodes, int nb, hipStream_t stream);
-void launch_safe_hhg(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 size_t safe_hh_lds_bytes(int nmax);
 void launch_safe_hh(NodeView* nodes, int nb, int nmax, hipStream_t stream);
 void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, const double* A64,

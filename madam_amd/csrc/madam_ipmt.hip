@@ -33,13 +33,12 @@
 // S2, V (Z).
 #include "madam_devutil.h"
 #include "madam_engine.h"
+#include "madam_tiles.h"
 
 namespace madam {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int IB = 32;      // block width of the factorisation and the inverse
-constexpr int RP = 64;      // panel rows per workgroup of a factorisation step
 constexpr int TR = 2;       // step lengths factored per line search and round
 constexpr int UPARTS = 64;  // workgroups per node of the update and its sums
 constexpr int IPM_MAXIT = 500;
@@ -51,17 +50,6 @@ struct IpmCtl {
   int fails_p, fails_d;   // candidates that failed so far
   int fin_p, fin_d;       // line search done
   int it, done, status, need;
-};
-
-// One Cholesky factorisation: A (n x n, ld n) factored in place, lower
-// triangle.  D holds the diagonal block factored in the latest step until
-// the next step writes it into A.
-struct CholJob {
-  double* A;
-  double* D;
-  int n, node;
-  int fail;  // 0, or 1 + the first column with a non-positive pivot
-  int skip;  // 1: nothing to factor in this sequence
 };
 
 __device__ __forceinline__ double rdlane(double v, int src) {
@@ -139,14 +127,13 @@ __device__ int chol_diag(double* Dl, int nbk, double* cb) {
 // FP64 MFMA lane layout (v_mfma_f64_16x16x4f64): A operand lane l -> (row
 // l % 16, k l / 16), B operand lane l -> (k l / 16, col l % 16), result
 // register v of lane l -> (row l / 16 + 4 v, col l % 16).
-__global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, const IpmCtl* ctl, int p,
-                                                   int npan) {
+__global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, int p, int npan) {
   __shared__ double Dl[IB * (IB + 1)];
   __shared__ double Rl[IB * (RP + 1)];
   __shared__ double cb[IB];
   __shared__ int bad;
   CholJob& J = jobs[blockIdx.y];
-  if (J.skip || J.fail || ctl[J.node].done) return;
+  if (J.skip || J.fail || (J.stop && *J.stop)) return;
   const int n = J.n, P = (n + IB - 1) / IB;
   if (p > P) return;
   double* __restrict__ A = J.A;
@@ -832,6 +819,17 @@ __global__ void k_ipmt_finish(NodeView* nodes, IpmCtl* ctl, const double* part, 
   nd.eig_warm = 0;
 }
 
+void chol_launches(CholJob* jobs, int nj, int nmax, hipStream_t st) {
+  const int P = (nmax + IB - 1) / IB;
+  for (int p = 0; p <= P; ++p) {
+    const int below = std::max(0, nmax - (p + 1) * IB);
+    const int npan = std::max(1, (below + RP - 1) / RP);
+    const int T = (below + 15) >> 4;
+    const int ntr = std::max(1, (T * (T + 1) / 2 + 3) / 4);
+    k_chol_step<<<dim3(npan + ntr, nj), 256, 0, st>>>(jobs, p, npan);
+  }
+}
+
 }  // namespace madam
 
 namespace mh {
@@ -839,7 +837,6 @@ namespace mh {
 using madam::CholJob;
 using madam::IB;
 using madam::IpmCtl;
-using madam::RP;
 using madam::TR;
 using madam::UPARTS;
 
@@ -859,17 +856,6 @@ void release_ipm_tiles() {
   g_hctl_cap = 0;
 }
 
-// All steps of a blocked factorisation of `nj` jobs (orders <= nmax).
-static void chol_launches(CholJob* jobs, int nj, const IpmCtl* ctl, int nmax, hipStream_t st) {
-  const int P = (nmax + IB - 1) / IB;
-  for (int p = 0; p <= P; ++p) {
-    const int below = std::max(0, nmax - (p + 1) * IB);
-    const int npan = std::max(1, (below + RP - 1) / RP);
-    const int T = (below + 15) >> 4;
-    const int ntr = std::max(1, (T * (T + 1) / 2 + 3) / 4);
-    madam::k_chol_step<<<dim3(npan + ntr, nj), 256, 0, st>>>(jobs, ctl, p, npan);
-  }
-}
 
 int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream_t st) {
   static_assert(TR == 2, "candidate buffers: ip2, ip3 (X) and S2, V (Z)");
@@ -886,13 +872,18 @@ int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream
     g_hctl_cap = nb;
   }
   std::vector<CholJob> J(njob);
-  for (int b = 0; b < nb; ++b) J[b] = {hv[b].ipP, g_iD.p + (size_t)b * IB * IB, hv[b].n, b, 0, 0};
+  // a job stops with its node: the address of ctl[b].done on the device
+  auto stop = [&](int b) {
+    return (const int*)((const char*)(g_ictl.p + b) + offsetof(IpmCtl, done));
+  };
+  for (int b = 0; b < nb; ++b)
+    J[b] = {hv[b].ipP, g_iD.p + (size_t)b * IB * IB, stop(b), hv[b].n, 0, 0, 0};
   for (int b = 0; b < nb; ++b)
     for (int s = 0; s < 2; ++s)
       for (int t = 0; t < TR; ++t) {
         const int q = (b * 2 + s) * TR + t, jq = nb + q;
         double* A = s == 0 ? (t == 0 ? hv[b].ip2 : hv[b].ip3) : (t == 0 ? hv[b].S2 : hv[b].V);
-        J[jq] = {A, g_iD.p + (size_t)jq * IB * IB, hv[b].n, b, 0, 1};
+        J[jq] = {A, g_iD.p + (size_t)jq * IB * IB, stop(b), hv[b].n, 0, 1, 0};
       }
   if ((rc = up(g_ijobs.p, J.data(), (size_t)njob, st))) return rc;
   CholJob* mjobs = g_ijobs.p;
@@ -918,7 +909,7 @@ int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream
     if (all) break;
     // Zi = Z^{-1}
     madam::k_ipmt_load<<<dim3(nload, nb), 256, 0, st>>>(d_nodes, ctl, mjobs, 0);
-    chol_launches(mjobs, nb, ctl, nmax, st);
+    madam::chol_launches(mjobs, nb, nmax, st);
     madam::k_ipmt_check<<<nbt, tpb, 0, st>>>(d_nodes, ctl, mjobs, nb, 1);
     for (int p = P - 1; p >= 0; --p) {
       const int m2 = std::max(0, nmax - (p + 1) * IB);
@@ -927,7 +918,7 @@ int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream
     madam::k_lauum<<<dim3((npair + 3) / 4, nb), 256, 0, st>>>(d_nodes, ctl);
     // predictor and corrector
     madam::k_ipmt_load<<<dim3(nload, nb), 256, 0, st>>>(d_nodes, ctl, mjobs, 1);
-    chol_launches(mjobs, nb, ctl, nmax, st);
+    madam::chol_launches(mjobs, nb, nmax, st);
     madam::k_ipmt_check<<<nbt, tpb, 0, st>>>(d_nodes, ctl, mjobs, nb, 3);
     madam::k_ipmt_solve<<<nb, 256, 0, st>>>(d_nodes, ctl, 0);
     madam::k_ipmt_gemm<<<dim3(npair, nb), 256, 0, st>>>(d_nodes, ctl, 0);
@@ -938,7 +929,7 @@ int ipm_tiles(NodeView* d_nodes, const NodeView* hv, int nb, int nmax, hipStream
     // step lengths
     for (int round = 0;; ++round) {
       madam::k_ipmt_trials<<<dim3(nload, 2 * TR * nb), 256, 0, st>>>(d_nodes, ctl, tjobs);
-      chol_launches(tjobs, 2 * TR * nb, ctl, nmax, st);
+      madam::chol_launches(tjobs, 2 * TR * nb, nmax, st);
       madam::k_ipmt_pick<<<nbt, tpb, 0, st>>>(ctl, tjobs, nb);
       if ((rc = read_ctl())) return rc;
       bool need = false;
