@@ -616,7 +616,10 @@ def main():
         # node of order 100 (tile-parallel layout): 6692 KiB -- the FP32
         # iterates pass through L2 between launches.  Algorithmic: W in, X+
         # out, 2 x 8 n^2 bytes.
-        traffic = {"c2-root": 6692.0 * 1024}.get(args.workload)
+        # HBM bytes per projection from rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # counter passes over this workload (fetch scaled x2, write x1, the
+        # factors a 64 MiB copy reads), summed over the projection's launches
+        traffic = {"c2-root": 6692.0 * 1024, "c5": 536144.0 * 1024}.get(args.workload)
         phase_ms = {p: round(float(v), 3) for p, v, c in zip(PHASES, tm.ms, tm.cnt) if c}
         roofline = {"bound": "mfma", "kernel": "sign-iteration projection (39 FP32-MFMA products + 3 "
                     "FP64-MFMA products; k_ns_gemm32/k_ns_gemm64 tile launches or k_ns_project)",
