@@ -168,6 +168,15 @@ int madam_time_projection(int n, int nb, int layout, int reps, double *ms);
 int madam_load_tuning(const char *path);
 int madam_tuned_layout(int n, int active);
 
+/* Testing hook, no reference counterpart: cap the LDS bytes the cut solve
+ * (PCG on (BB^T + I) t = rhs) plans with, so that small problems run the
+ * kernels otherwise chosen only for large cut sets (vectors partly or
+ * wholly in HBM), and with one_pass = 0 keep nodes off the one-pass kernel
+ * (products over precomputed expanded rows).  (0, 1) restores the
+ * defaults.  Host only; applies to the calls that follow.  All kernels
+ * solve the same system and the solutions agree to the PCG tolerance. */
+int madam_set_cg_lds_cap(long bytes, int one_pass);
+
 /* Coefficients (a_k, b_k, c_k) of the sign iteration's steps
  * x -> a x + b x^3 + c x^5 used by the mixed-precision projection (no
  * reference counterpart): writes min(cap, steps) triples to coef and

@@ -121,6 +121,13 @@ def tuned_layout(n, active):
     return int(lib().madam_tuned_layout(C.c_int(n), C.c_int(active)))
 
 
+def set_cg_lds_cap(nbytes, one_pass=True):
+    """Cap the LDS bytes the cut solve plans with (0: the device's), so that
+    small problems run the kernels meant for large cut sets, and with
+    one_pass=False keep nodes off the one-pass kernel (testing)."""
+    _check(lib().madam_set_cg_lds_cap(C.c_long(nbytes), C.c_int(1 if one_pass else 0)))
+
+
 def _f64(a):
     return np.asfortranarray(a, dtype=np.float64)
 

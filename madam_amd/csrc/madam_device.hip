@@ -16,6 +16,8 @@
 
 namespace madam {
 
+static_assert(XCUT_THREADS == WG, "the expanded rows are assigned to the cut solve's threads");
+
 // ------------------------------------------------------------ ADMM steps
 
 // y = diag(L+Q+X/sigma) - e/sigma ; rhs = s + B(L+Q+X/sigma) + (u-e)/sigma
@@ -70,15 +72,21 @@ __device__ void spmv_bbt(const NodeView& nd, const double* __restrict__ p, doubl
 }
 
 // Which cut-solve kernel serves a node: 0 k_cg1<2> (m <= 2 WG), 1 k_cg1<4>
-// (m <= 4 WG), 2 k_cg_lds, 3 k_cg (the LDS vectors do not fit).  Decided
-// per node, so that one large node does not move a whole batch to a
-// slower kernel; every cut-solve launch skips the nodes of other classes.
-__host__ __device__ inline int cg_class(int m, int nact, size_t lds_limit) {
+// (m <= 4 WG; both with the cut and pair vectors in LDS), 2 k_cg_x (one-pass
+// products over the sliced Gram rows, when built, with the +-ps table in
+// LDS), 3
+// k_cg_big<true> (cut and pair vectors in LDS), 4 k_cg_big<false> (only the
+// cut vector fits), 5 k_cg (neither fits).  Decided per node, so that one
+// large node does not move a whole batch to a slower kernel; every
+// cut-solve launch skips the nodes of other classes.
+__host__ __device__ inline int cg_class(int m, int nact, int xrows, size_t lds_limit) {
   const size_t base = ((size_t)m + nact + 64) * sizeof(double);
-  if (base > lds_limit) return 3;
-  if (m <= 2 * WG) return 0;
-  if (m <= 4 * WG) return 1;
-  return 2;
+  const size_t cut_only = ((size_t)m + 64) * sizeof(double);
+  if (base <= lds_limit && m <= 4 * WG) return m <= 2 * WG ? 0 : 1;
+  // k_cg_x: the +-ps table (2 m + 2 doubles) and 64 reduction doubles
+  if (xrows && (2 * (size_t)m + 66) * sizeof(double) <= lds_limit) return 2;
+  if (base <= lds_limit) return 3;
+  return cut_only <= lds_limit ? 4 : 5;
 }
 
 // Stopping tolerance of a cut solve: rtol, or with adapt > 0 after the
@@ -95,10 +103,10 @@ __device__ __forceinline__ double cg_tol(const NodeView& nd, double rtol, double
 // relative residual of rtol.  Replaces the CHOLMOD factor/solve of
 // admm.c:486-551.
 __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double adapt, int maxit,
-                                           size_t lds_limit) {
+                                           size_t lds_limit, int use_x) {
   __shared__ double red[4 * 16];
   NodeView& nd = nodes[blockIdx.x];
-  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != 3) return;
+  if (nd.done || cg_class(nd.m, nd.nact, use_x ? nd.xrows : 0, lds_limit) != 5) return;
   const int m = nd.m;
   if (m == 0) return;
   double* __restrict__ x = nd.t;
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double 
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
-  // starting guess extrapolated from the previous solutions (as k_cg_lds)
+  // starting guess extrapolated from the previous solutions (as k_cg_big)
   for (int c = threadIdx.x; c < m; c += WG) {
     const double tk = x[c], t1 = nd.tprev[c];
     if (nd.nbit > 1)
@@ -167,81 +175,141 @@ __global__ __launch_bounds__(WG) void k_cg(NodeView* nodes, double rtol, double 
   }
 }
 
-// The same PCG with the cut-side vector ps = scale * v and the pair vector
-// w in LDS (when m + nact doubles fit): both passes of the product then
-// gather from LDS and stream only their index arrays (aent, cact) from HBM.
-// Same operations in the same order as k_cg, so the iterates are
-// bit-identical.
-template <class Idx>
-__device__ inline void bbt_lds(const NodeView& nd, const Idx* __restrict__ aent,
-                               const Idx* __restrict__ cact, const double* __restrict__ v,
-                               double* __restrict__ out, double* ps, double* w) {
-  const int m = nd.m, na = nd.nact;
-  for (int c = threadIdx.x; c < m; c += WG) ps[c] = nd.cscale[c] * v[c];
-  __syncthreads();
-  for (int a = threadIdx.x; a < na; a += WG) {
-    double acc = 0.0;
-    const int e1 = nd.aptr[a + 1];
-    for (int e = nd.aptr[a]; e < e1; ++e) {
-      const int en = aent[e];
-      const double x = ps[ent_idx(en)];
-      acc = en >= 0 ? acc + x : acc - x;
+// Signed sums over S index ranges at once: acc[s] (+/-) v[ent_idx(en)] over
+// the entries en of ent[e[s]..e1[s]), each range in entry order (so each sum
+// is the same bits as the one-at-a-time loop).  G entries of every range,
+// then their values, are loaded before any is added -- S * G loads in
+// flight -- and a range past its end reloads its last entry (or entry 0 of
+// ent / v when empty; both must be non-empty) whose value is not added.
+template <int S, int G, class Idx>
+__device__ __forceinline__ void signed_sums(const Idx* __restrict__ ent, const double* v,
+                                            int (&e)[S], const int (&e1)[S], double (&acc)[S]) {
+  bool more = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) more = more || e[s] < e1[s];
+  while (more) {
+    int k[S][G];
+    double x[S][G];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int q = 0; q < G; ++q)
+        k[s][q] = e[s] < e1[s] ? (int)ent[e[s] + q < e1[s] ? e[s] + q : e1[s] - 1] : 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int q = 0; q < G; ++q) x[s][q] = v[ent_idx(k[s][q])];
+    more = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const double t = k[s][q] >= 0 ? acc[s] + x[s][q] : acc[s] - x[s][q];
+        acc[s] = e[s] + q < e1[s] ? t : acc[s];
+      }
+      e[s] += G;
+      more = more || e[s] < e1[s];
     }
-    w[a] = acc;
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < m; c += WG) {
-    double acc = 0.0;
-    const int e1 = nd.cptr[c + 1];
-    for (int e = nd.cptr[c]; e < e1; ++e) {
-      const int ca = cact[e];
-      const double x = w[ent_idx(ca)];
-      acc = ca >= 0 ? acc + x : acc - x;
-    }
-    out[c] = v[c] + 0.5 * nd.cscale[c] * acc;
   }
 }
 
+// q = (BB^T + I) v for the cut solves of one workgroup, with the cut-side
+// vector ps = scale * v in LDS and the pair sums w in LDS (WLDS) or in HBM
+// (nd.cg_w, when m + nact doubles do not fit).  The pair pass takes four
+// pairs per thread at a time; the cut pass gives each thread cut q together
+// with cut m-1-q (triangles, with 3 entries, come first and heptagonals,
+// with 21, last), two rows in flight.  Every sum is in the entry order of
+// the one-at-a-time loops, so the products are bit-identical to k_cg's
+// spmv_bbt.  On entry ps must hold scale * v and be visible to the
+// workgroup; on return q is, to the whole workgroup.
 template <class Idx>
-__device__ void cg_lds_body(NodeView& nd, double rtol, double adapt, int maxit) {
+__device__ __forceinline__ void bbt_big(const NodeView& nd, const Idx* __restrict__ aent,
+                                        const Idx* __restrict__ cact, const double* __restrict__ v,
+                                        double* __restrict__ out, const double* ps, double* w) {
+  const int m = nd.m, na = nd.nact, tid = threadIdx.x;
+  const int* __restrict__ aptr = nd.aptr;
+  const int* __restrict__ cptr = nd.cptr;
+  for (int a = tid; a < na; a += 4 * WG) {
+    int e[4], e1[4];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int as = a + s * WG;
+      e[s] = as < na ? aptr[as] : 0;
+      e1[s] = as < na ? aptr[as + 1] : 0;
+    }
+    signed_sums<4, 2>(aent, ps, e, e1, acc);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (a + s * WG < na) w[a + s * WG] = acc[s];
+  }
+  __syncthreads();
+  const int half = (m + 1) >> 1;
+  for (int q = tid; q < half; q += WG) {
+    const int c2 = m - 1 - q;
+    const bool two = c2 > q;
+    int e[2] = {cptr[q], two ? cptr[c2] : 0};
+    const int e1[2] = {cptr[q + 1], two ? cptr[c2 + 1] : 0};
+    double acc[2] = {0.0, 0.0};
+    signed_sums<2, 4>(cact, w, e, e1, acc);
+    out[q] = v[q] + 0.5 * nd.cscale[q] * acc[0];
+    if (two) out[c2] = v[c2] + 0.5 * nd.cscale[c2] * acc[1];
+  }
+  __syncthreads();
+}
+
+// The PCG of k_cg for nodes whose cut-side vector fits LDS: ps (and, with
+// WLDS, the pair sums) in LDS, the index streams (16-bit when m, nact <
+// 2^15) and the PCG vectors in HBM; products by bbt_big.  The update of p
+// is fused into the pass that forms the next product's ps.  Same
+// operations in the same order as k_cg: the iterates are bit-identical.
+template <class Idx, bool WLDS>
+__device__ void cg_big_body(NodeView& nd, double rtol, double adapt, int maxit) {
   extern __shared__ double smem[];
-  // 16-bit index streams halve the bytes each product reads (the launcher
-  // picks them only when every node has m, nact < 2^15)
   const Idx* aent = sizeof(Idx) == 2 ? (const Idx*)nd.aent16 : (const Idx*)nd.aent;
   const Idx* cact = sizeof(Idx) == 2 ? (const Idx*)nd.cact16 : (const Idx*)nd.cact;
   if (!aent) return;
-  const int m = nd.m;
+  const int m = nd.m, tid = threadIdx.x;
   if (m == 0) return;
   double* ps = smem;
-  double* w = smem + m;
-  double* red = w + nd.nact;
+  double* w = WLDS ? smem + m : nd.cg_w;
+  double* red = smem + m + (WLDS ? nd.nact : 0);
   double* __restrict__ x = nd.t;
   double* __restrict__ r = nd.cg_r;
   double* __restrict__ p = nd.cg_p;
   double* __restrict__ q = nd.cg_q;
   const double* __restrict__ b = nd.rhs;
+  const int* __restrict__ cptr = nd.cptr;
+  const double* __restrict__ cs = nd.cscale;
   // starting guess: the previous solutions extrapolated -- quadratically,
   // 3 t_k - 3 t_{k-1} + t_{k-2}, from the third iteration of an ADMM call
   // on, linearly at the second (the iterates move smoothly, so the first
   // residual is smaller)
-  for (int c = threadIdx.x; c < m; c += WG) {
+#pragma unroll 4
+  for (int c = tid; c < m; c += WG) {
     const double tk = x[c], t1 = nd.tprev[c];
+    double xc = tk;
     if (nd.nbit > 1)
-      x[c] = 3.0 * (tk - t1) + nd.tprev2[c];
+      xc = 3.0 * (tk - t1) + nd.tprev2[c];
     else if (nd.nbit > 0)
-      x[c] = tk + (tk - t1);
+      xc = tk + (tk - t1);
+    x[c] = xc;
     nd.tprev2[c] = t1;
     nd.tprev[c] = tk;
+    ps[c] = cs[c] * xc;
   }
   __syncthreads();
-  bbt_lds(nd, aent, cact, x, q, ps, w);
+  bbt_big(nd, aent, cact, x, q, ps, w);
   double v3[3] = {0, 0, 0};  // r.z, r.r, b.b
-  for (int c = threadIdx.x; c < m; c += WG) {
-    const int k = nd.cptr[c + 1] - nd.cptr[c];
-    const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
+#pragma unroll 4
+  for (int c = tid; c < m; c += WG) {
+    const int k = cptr[c + 1] - cptr[c];
+    const double dinv = 1.0 / (1.0 + 0.5 * k * cs[c] * cs[c]);
     const double rc = b[c] - q[c];
     r[c] = rc;
-    p[c] = rc * dinv;
+    const double pc = rc * dinv;
+    p[c] = pc;
+    ps[c] = cs[c] * pc;
     v3[0] += rc * rc * dinv;
     v3[1] += rc * rc;
     v3[2] += b[c] * b[c];
@@ -253,15 +321,17 @@ __device__ void cg_lds_body(NodeView& nd, double rtol, double adapt, int maxit) 
   const double thr = rt * rt * bb;
   int it = 0;
   while (it < maxit && rr > thr && bb > 0.0) {
-    bbt_lds(nd, aent, cact, p, q, ps, w);
+    bbt_big(nd, aent, cact, p, q, ps, w);
     double v1[1] = {0.0};
-    for (int c = threadIdx.x; c < m; c += WG) v1[0] += p[c] * q[c];
+#pragma unroll 4
+    for (int c = tid; c < m; c += WG) v1[0] += p[c] * q[c];
     block_sums<1>(v1, red);
     const double alpha = rz / v1[0];
     double v2[2] = {0, 0};
-    for (int c = threadIdx.x; c < m; c += WG) {
-      const int k = nd.cptr[c + 1] - nd.cptr[c];
-      const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
+#pragma unroll 4
+    for (int c = tid; c < m; c += WG) {
+      const int k = cptr[c + 1] - cptr[c];
+      const double dinv = 1.0 / (1.0 + 0.5 * k * cs[c] * cs[c]);
       x[c] += alpha * p[c];
       const double rc = r[c] - alpha * q[c];
       r[c] = rc;
@@ -272,28 +342,33 @@ __device__ void cg_lds_body(NodeView& nd, double rtol, double adapt, int maxit) 
     const double beta = v2[0] / rz;
     rz = v2[0];
     rr = v2[1];
-    for (int c = threadIdx.x; c < m; c += WG) {
-      const int k = nd.cptr[c + 1] - nd.cptr[c];
-      const double dinv = 1.0 / (1.0 + 0.5 * k * nd.cscale[c] * nd.cscale[c]);
-      p[c] = r[c] * dinv + beta * p[c];
+    ++it;
+    if (!(it < maxit && rr > thr)) break;
+#pragma unroll 4
+    for (int c = tid; c < m; c += WG) {
+      const int k = cptr[c + 1] - cptr[c];
+      const double dinv = 1.0 / (1.0 + 0.5 * k * cs[c] * cs[c]);
+      const double pc = r[c] * dinv + beta * p[c];
+      p[c] = pc;
+      ps[c] = cs[c] * pc;
     }
     __syncthreads();
-    ++it;
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     nd.cg_iters = it;
     nd.cg_total += it;
   }
 }
 
-__global__ __launch_bounds__(WG) void k_cg_lds(NodeView* nodes, double rtol, double adapt, int maxit,
-                                               size_t lds_limit) {
+template <bool WLDS>
+__global__ __launch_bounds__(WG) void k_cg_big(NodeView* nodes, double rtol, double adapt, int maxit,
+                                               size_t lds_limit, int use_x) {
   NodeView& nd = nodes[blockIdx.x];
-  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != 2) return;
+  if (nd.done || cg_class(nd.m, nd.nact, use_x ? nd.xrows : 0, lds_limit) != (WLDS ? 3 : 4)) return;
   if (nd.aent16)
-    cg_lds_body<short>(nd, rtol, adapt, maxit);
+    cg_big_body<short, WLDS>(nd, rtol, adapt, maxit);
   else
-    cg_lds_body<int>(nd, rtol, adapt, maxit);
+    cg_big_body<int, WLDS>(nd, rtol, adapt, maxit);
 }
 
 // sum of +-v[idx] over the signed entries ent[e0..e1) in entry order,
@@ -522,13 +597,276 @@ __device__ void cg1_body(NodeView& nd, double rtol, double adapt, int maxit, siz
 
 template <int CPT>
 __global__ __launch_bounds__(WG) void k_cg1(NodeView* nodes, double rtol, double adapt, int maxit,
-                                            size_t lds_limit, size_t lds_alloc) {
+                                            size_t lds_limit, size_t lds_alloc, int use_x) {
   NodeView& nd = nodes[blockIdx.x];
-  if (nd.done || cg_class(nd.m, nd.nact, lds_limit) != (CPT == 2 ? 0 : 1)) return;
+  if (nd.done || cg_class(nd.m, nd.nact, use_x ? nd.xrows : 0, lds_limit) != (CPT == 2 ? 0 : 1)) return;
   if (nd.aent16)
     cg1_body<CPT, short>(nd, rtol, adapt, maxit, lds_alloc);
   else
     cg1_body<CPT, int>(nd, rtol, adapt, maxit, lds_alloc);
+}
+
+// Sums of the sliced Gram rows (NodeView::xent) of two slots of the
+// calling lane at once: row s has its words at xe[f[s] + 64 q] for q <
+// L[s] (its slice's length, a multiple of 8, rows padded with the zero
+// entry's offset); each word is the byte offset in the LDS table `tab` of
+// the signed entry to add to acc[s], in row order.  The loop runs to L[0]
+// >= L[1], row 1 only while it lasts (both the same on every lane of a
+// wave); 8 words of each row in flight, each load one contiguous access of
+// the wave.
+__device__ __forceinline__ void x_rows2(const GMEM unsigned* __restrict__ xe, const char* tab,
+                                        const int (&f)[2], const int (&L)[2], double (&acc)[2]) {
+  constexpr int G = 8;
+  for (int q0 = 0; q0 < L[0]; q0 += G) {
+    unsigned wa[G], wb[G];
+    const bool two = q0 < L[1];
+#pragma unroll
+    for (int q = 0; q < G; ++q) wa[q] = xe[f[0] + 64 * (q0 + q)];
+    if (two) {
+#pragma unroll
+      for (int q = 0; q < G; ++q) wb[q] = xe[f[1] + 64 * (q0 + q)];
+    }
+    double va[G], vb[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) va[q] = *(const double*)(tab + wa[q]);
+    if (two) {
+#pragma unroll
+      for (int q = 0; q < G; ++q) vb[q] = *(const double*)(tab + wb[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) acc[0] += va[q];
+    if (two) {
+#pragma unroll
+      for (int q = 0; q < G; ++q) acc[1] += vb[q];
+    }
+  }
+}
+
+// The cut solve in the Chronopoulos-Gear form of k_cg1 with one-pass
+// products over the sliced Gram rows of BB^T.  Slot i (cut tcut[i], rows
+// longest first) belongs to thread i % WG; the PCG vectors x, r, p, s and
+// w = A u live in HBM in slot order (every elementwise access of a wave is
+// contiguous), with the scale and the diagonal preconditioner of each
+// slot; the LDS holds the table +-ps (ps = scale * u, entries 2c and 2c+1
+// for cut c, entry 2m zero) that the rows address.  The solution is
+// gathered from and scattered to t (cut order) once per solve, and
+// tprev/tprev2 are kept in slot order.  Each thread forms the products of
+// its slots two at a time: (A u)_c = u_c + scale_c / 2 (k_c ps_c + row sum).
+// Per iteration: one elementwise pass, one barrier, the product with the
+// three dot products, one reduction (one barrier).  Same linear system,
+// start and stopping rule as k_cg; the iterates agree with it to rounding.
+__global__ __launch_bounds__(WG) void k_cg_x(NodeView* nodes, double rtol, double adapt, int maxit,
+                                             size_t lds_limit) {
+  NodeView& nd = nodes[blockIdx.x];
+  if (nd.done || cg_class(nd.m, nd.nact, nd.xrows, lds_limit) != 2) return;
+  extern __shared__ double smem[];
+  const int m = nd.m, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double* tab = smem;
+  double* red = smem + 2 * m + 2;
+  const GMEM int* __restrict__ xoff = gmem(nd.xoff);
+  const GMEM unsigned* __restrict__ xe = gmem(nd.xent);
+  const GMEM int* __restrict__ tcut = gmem(nd.tcut);
+  const GMEM int* __restrict__ tmeta = gmem(nd.tmeta);
+  const GMEM double* __restrict__ tsc = gmem(nd.tsc);
+  const GMEM double* __restrict__ tdv = gmem(nd.tdinv);
+  const GMEM double* __restrict__ b = gmem(nd.rhs);
+  GMEM double* __restrict__ t = gmem(nd.t);
+  GMEM double* __restrict__ x = gmem(nd.xs);
+  GMEM double* __restrict__ r = gmem(nd.cg_r);
+  GMEM double* __restrict__ p = gmem(nd.cg_p);
+  GMEM double* __restrict__ s = gmem(nd.cg_q);
+  GMEM double* __restrict__ w = gmem(nd.cg_w);
+  GMEM double* __restrict__ tp1 = gmem(nd.tprev);
+  GMEM double* __restrict__ tp2 = gmem(nd.tprev2);
+  if (tid == 0) tab[2 * m] = 0.0;
+  // this thread's slots: tid + k WG for k < ks
+  const int ks = m > tid ? (m - tid + WG - 1) / WG : 0;
+  // reduction with one barrier: every call is separated from the previous
+  // one by the barrier that publishes the table, so no thread still reads red
+  auto reduce = [&](auto& v) {
+    constexpr int NV = sizeof(v) / sizeof(double);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) red[q * 16 + wv] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double a = 0.0;
+      for (int i = 0; i < WG / 64; ++i) a += red[q * 16 + i];
+      v[q] = a;
+    }
+  };
+  auto put = [&](int c, double v) {
+    tab[2 * c] = v;
+    tab[2 * c + 1] = -v;
+  };
+  // per-slot operands loaded ahead of use
+  struct Ops {
+    double a, b, c, d, e, sc, di;
+    int k, cut;
+  };
+  // this thread's slots four at a time: ld(i, o) for all four (a step past
+  // the thread's share loads its first slot's operands, unused), then
+  // st(i, o) for the real ones
+  auto forall4 = [&](auto ld, auto st) {
+    for (int k = 0; k < ks; k += 4) {
+      Ops o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ld(tid + (k + q < ks ? k + q : 0) * WG, o[q]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k + q < ks) st(tid + (k + q) * WG, o[q]);
+    }
+  };
+  // this thread's slots two at a time through the product: ld(i, o) before
+  // the row sums, u(o) the slot's u, fin(i, o, acc) after with acc =
+  // k ps + the row sum
+  auto products = [&](auto ld, auto u, auto fin) {
+    for (int k = 0; k < ks; k += 2) {
+      const bool two = k + 1 < ks;
+      const int i0 = tid + k * WG, i1 = two ? i0 + WG : i0;
+      Ops o[2];
+      ld(i0, o[0]);
+      ld(i1, o[1]);
+      const int s0 = i0 >> 6, s1 = i1 >> 6;
+      const int f[2] = {xoff[s0] + lane, xoff[s1] + lane};
+      const int L[2] = {(xoff[s0 + 1] - xoff[s0]) >> 6,
+                        two ? (xoff[s1 + 1] - xoff[s1]) >> 6 : 0};
+      double acc[2] = {o[0].k * (o[0].sc * u(o[0])), o[1].k * (o[1].sc * u(o[1]))};
+      x_rows2(xe, (const char*)tab, f, L, acc);
+      fin(i0, o[0], acc[0]);
+      if (two) fin(i1, o[1], acc[1]);
+    }
+  };
+  auto ld_r = [&](int i, Ops& o) {
+    o.a = r[i];
+    o.sc = tsc[i];
+    o.di = tdv[i];
+    o.k = tmeta[i];
+    o.cut = tcut[i];
+  };
+  auto u_r = [](const Ops& o) { return o.a * o.di; };
+  // starting guess extrapolated from the previous solutions (as k_cg)
+  const int nbit = nd.nbit;
+  forall4(
+      [&](int i, Ops& o) {
+        o.cut = tcut[i];
+        o.a = t[o.cut];
+        o.b = tp1[i];
+        o.c = tp2[i];
+        o.sc = tsc[i];
+      },
+      [&](int i, Ops& o) {
+        const double tk = o.a, t1 = o.b;
+        double xc = tk;
+        if (nbit > 1)
+          xc = 3.0 * (tk - t1) + o.c;
+        else if (nbit > 0)
+          xc = tk + (tk - t1);
+        x[i] = xc;
+        tp2[i] = t1;
+        tp1[i] = tk;
+        put(o.cut, o.sc * xc);
+      });
+  __syncthreads();
+  double v3[3] = {0, 0, 0};  // (r,u), (r,r), (b,b)
+  products(
+      [&](int i, Ops& o) {
+        o.cut = tcut[i];
+        o.a = x[i];
+        o.b = b[o.cut];
+        o.sc = tsc[i];
+        o.di = tdv[i];
+        o.k = tmeta[i];
+      },
+      [](const Ops& o) { return o.a; },
+      [&](int i, Ops& o, double acc) {
+        const double rc = o.b - (o.a + 0.5 * o.sc * acc);
+        r[i] = rc;
+        const double uc = rc * o.di;
+        v3[0] += rc * uc;
+        v3[1] += rc * rc;
+        v3[2] += o.b * o.b;
+      });
+  reduce(v3);
+  double gamma = v3[0], rr = v3[1];
+  const double bb = v3[2];
+  const double rt = cg_tol(nd, rtol, adapt);
+  const double thr = rt * rt * bb;
+  double alpha = 0.0, beta = 0.0;
+  int it = 0;
+  bool run = it < maxit && rr > thr && bb > 0.0;
+  if (run) {
+    // w0 = A u0, alpha0 = (r0,u0) / (w0,u0)
+    forall4(ld_r, [&](int i, Ops& o) {
+      put(o.cut, o.sc * u_r(o));
+      p[i] = 0.0;
+      s[i] = 0.0;
+    });
+    __syncthreads();
+    double v1[1] = {0.0};
+    products(ld_r, u_r, [&](int i, Ops& o, double acc) {
+      const double uc = u_r(o);
+      const double wc = uc + 0.5 * o.sc * acc;
+      w[i] = wc;
+      v1[0] += wc * uc;
+    });
+    reduce(v1);
+    alpha = gamma / v1[0];
+  }
+  while (run) {
+    forall4(
+        [&](int i, Ops& o) {
+          o.a = r[i];
+          o.b = p[i];
+          o.c = w[i];
+          o.d = s[i];
+          o.e = x[i];
+          o.sc = tsc[i];
+          o.di = tdv[i];
+          o.cut = tcut[i];
+        },
+        [&](int i, Ops& o) {
+          const double pc = o.a * o.di + beta * o.b;
+          const double sc = o.c + beta * o.d;
+          p[i] = pc;
+          s[i] = sc;
+          x[i] = o.e + alpha * pc;
+          const double rc = o.a - alpha * sc;
+          r[i] = rc;
+          put(o.cut, o.sc * (rc * o.di));
+        });
+    __syncthreads();
+    double v[3] = {0, 0, 0};  // (r,u), (w,u), (r,r)
+    products(ld_r, u_r, [&](int i, Ops& o, double acc) {
+      const double rc = o.a;
+      const double uc = u_r(o);
+      const double wc = uc + 0.5 * o.sc * acc;
+      w[i] = wc;
+      v[0] += rc * uc;
+      v[1] += wc * uc;
+      v[2] += rc * rc;
+    });
+    reduce(v);
+    ++it;
+    rr = v[2];
+    beta = v[0] / gamma;
+    alpha = v[0] / (v[1] - beta * v[0] / alpha);
+    gamma = v[0];
+    run = it < maxit && rr > thr;
+  }
+  // the solution back to cut order
+  for (int k = 0; k < ks; ++k) {
+    const int i = tid + k * WG;
+    t[tcut[i]] = x[i];
+  }
+  if (tid == 0) {
+    nd.cg_iters = it;
+    nd.cg_total += it;
+  }
 }
 
 // Cut duals (admm.c:557-560, 589-591) and the matrix to project
@@ -1008,26 +1346,37 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
 }
 
 void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
-               size_t lds_limit, hipStream_t st) {
+               size_t lds_limit, int use_x, hipStream_t st) {
   // per class: is any node in it, and the LDS its largest node asks for
   // (vectors, plus the staged index streams when those fit)
-  size_t need[4] = {0, 0, 0, 0};
-  bool any[4] = {false, false, false, false};
+  size_t need[6] = {0, 0, 0, 0, 0, 0};
+  bool any[6] = {false, false, false, false, false, false};
   for (int b = 0; b < nb; ++b) {
     const NodeView& v = hv[b];
     if (v.done || v.m == 0) continue;
-    const int cls = cg_class(v.m, v.nact, lds_limit);
+    const int cls = cg_class(v.m, v.nact, use_x ? v.xrows : 0, lds_limit);
     any[cls] = true;
-    const size_t base = ((size_t)v.m + v.nact + 64) * sizeof(double);
-    const size_t staged = base + ((size_t)v.nact + 1) * sizeof(int) +
-                          2 * (size_t)v.ne * (v.aent16 ? sizeof(short) : sizeof(int));
-    const size_t want = cls <= 1 && staged <= lds_limit ? staged : base;
-    if (want > need[cls]) need[cls] = want;
+    // LDS: the cut vector (and the pair vector where it lives there) and 64
+    // reduction doubles, plus for k_cg1 the index streams when they fit
+    const size_t base = cls == 2 ? (2 * (size_t)v.m + 66) * sizeof(double)
+                                 : ((size_t)v.m + (cls == 4 ? 0 : v.nact) + 64) * sizeof(double);
+    size_t staged = base;
+    if (cls <= 1)
+      staged += ((size_t)v.nact + 1) * sizeof(int) +
+                2 * (size_t)v.ne * (v.aent16 ? sizeof(short) : sizeof(int));
+    const size_t want = staged <= lds_limit ? staged : base;
+    if (cls < 5 && want > need[cls]) need[cls] = want;
   }
-  if (any[0]) k_cg1<2><<<nb, WG, need[0], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[0]);
-  if (any[1]) k_cg1<4><<<nb, WG, need[1], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[1]);
-  if (any[2]) k_cg_lds<<<nb, WG, need[2], st>>>(nodes, rtol, adapt, maxit, lds_limit);
-  if (any[3]) k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit, lds_limit);
+  const int ux = use_x ? 1 : 0;
+  if (any[0])
+    k_cg1<2><<<nb, WG, need[0], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[0], ux);
+  if (any[1])
+    k_cg1<4><<<nb, WG, need[1], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[1], ux);
+  // k_cg_x runs only with use_x (any[2] is false otherwise)
+  if (any[2]) k_cg_x<<<nb, WG, need[2], st>>>(nodes, rtol, adapt, maxit, lds_limit);
+  if (any[3]) k_cg_big<true><<<nb, WG, need[3], st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
+  if (any[4]) k_cg_big<false><<<nb, WG, need[4], st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
+  if (any[5]) k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

@@ -11,6 +11,17 @@ namespace madam {
 constexpr int WG = 1024;   // per-node kernels (16 waves)
 constexpr int WGS = 256;   // streaming kernels
 
+// Pointers known to address global memory.  Loads and stores through a
+// pointer read out of a NodeView compile to flat_* instructions, whose
+// completion the LDS wait counter tracks too (so a wait for an LDS read also
+// drains every global load in flight) and which take 64-bit VGPR addresses;
+// through a GMEM pointer they are global_* (SGPR base + VGPR offset).
+#define GMEM __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GMEM T* gmem(T* p) {
+  return (GMEM T*)p;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -65,7 +65,84 @@ struct CutSet {
   std::vector<int> cptr, cent, cact, aptr, aent, apair, p2a;
   std::vector<short> aent16, cact16;  // the same streams in 16 bits when m, nact < 2^15
   std::vector<double> cscale, y0;
+  // sliced Gram rows of the one-pass cut solve (see build_xrows); empty
+  // when not built
+  std::vector<int> xoff, tcut, tmeta;
+  std::vector<double> tsc, tdinv;
+  std::vector<unsigned> xent;
 };
+
+// Rows of the Gram matrix BB^T for the one-pass product of the cut solve,
+// sliced for coalesced loads.  (BB^T)_cc' = scale_c scale_c' G_cc' with
+// G_cc' = sum over the pairs shared by c and c' of the product of their
+// signs there, G_cc = k_c (the pair entries of c).  Row c lists, for each
+// pair entry of c in entry order and each other cut c' of that pair in
+// increasing cut order, one word: the LDS byte offset of entry 2 c' + neg
+// of a table holding +-scale_c' u_c' (neg when the two signs differ), so
+// that the product is q_c = v_c + scale_c / 2 (k_c ps_c + sum of the row's
+// table entries) with no sign or pair logic per word.  The cuts are sorted
+// by row length, longest first (ties by cut), into slots: slot i holds cut
+// tcut[i] and belongs to thread i % XCUT_THREADS of the solve.  Slice s =
+// slots 64 s .. 64 s + 63 (one wave's lanes at one step) stores word q of
+// lane l's row at xoff[s] + 64 q + l, its rows padded to the longest
+// rounded up to 8 with the offset 16 m of the table's zero entry.  tmeta[i]
+// = pair entries of the cut, tsc[i] its scale and tdinv[i] the diagonal
+// preconditioner 1 / (1 + k scale^2 / 2).  Not built (empty) when m >= 2^16
+// or the sliced rows exceed 16 words per pair entry plus 4096.
+inline void build_xrows(CutSet& cs) {
+  cs.xoff.clear();
+  cs.xent.clear();
+  cs.tcut.clear();
+  cs.tmeta.clear();
+  cs.tsc.clear();
+  cs.tdinv.clear();
+  const int m = cs.m;
+  if (m == 0 || m >= 65536) return;
+  const size_t ne = cs.cent.size();
+  std::vector<int> len(m, 0);
+  for (int c = 0; c < m; ++c)
+    for (int e = cs.cptr[c]; e < cs.cptr[c + 1]; ++e) {
+      const int a = cs.cact[e] >= 0 ? cs.cact[e] : ~cs.cact[e];
+      len[c] += cs.aptr[a + 1] - cs.aptr[a] - 1;
+    }
+  std::vector<int> order(m);
+  for (int c = 0; c < m; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len[a] > len[b]; });
+  const int ns = (m + 63) / 64;
+  std::vector<int> off(ns + 1, 0);
+  for (int sl = 0; sl < ns; ++sl) {
+    off[sl + 1] = off[sl] + 64 * ((len[order[64 * sl]] + 7) & ~7);
+    if ((size_t)off[sl + 1] > 16 * ne + 4096) return;
+  }
+  cs.xoff.swap(off);
+  const unsigned zero = 16u * (unsigned)m;
+  cs.xent.assign(cs.xoff[ns], zero);
+  cs.tcut = order;
+  cs.tmeta.resize(m);
+  cs.tsc.resize(m);
+  cs.tdinv.resize(m);
+  for (int i = 0; i < m; ++i) {
+    const int c = order[i];
+    const int k = cs.cptr[c + 1] - cs.cptr[c];
+    const double sc = cs.cscale[c];
+    cs.tmeta[i] = k;
+    cs.tsc[i] = sc;
+    cs.tdinv[i] = 1.0 / (1.0 + 0.5 * k * sc * sc);
+    size_t f = (size_t)cs.xoff[i / 64] + (i % 64);
+    for (int e = cs.cptr[c]; e < cs.cptr[c + 1]; ++e) {
+      const int ca = cs.cact[e];
+      const int a = ca >= 0 ? ca : ~ca;
+      for (int g = cs.aptr[a]; g < cs.aptr[a + 1]; ++g) {
+        const int en = cs.aent[g];
+        const int c2 = en >= 0 ? en : ~en;
+        if (c2 == c) continue;
+        const unsigned neg = (en < 0) != (ca < 0) ? 1u : 0u;
+        cs.xent[f] = 8u * (2u * (unsigned)c2 + neg);
+        f += 64;
+      }
+    }
+  }
+}
 
 // Vertex tuple and sign vector (operators.c:85-148): triangle types 2/3/4
 // negate vertex k/j/i; pentagonal/heptagonal type t negates the first t-1.
@@ -193,6 +270,7 @@ inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const 
     cs.aent16.assign(cs.aent.begin(), cs.aent.end());
     cs.cact16.assign(cs.cact.begin(), cs.cact.end());
   }
+  build_xrows(cs);
   return MADAM_OK;
 }
 
@@ -242,9 +320,10 @@ struct DBuf {
 
 struct Slot {
   DBuf<double> L, X, Q, W, V, S1, S2, ev, evp, y;
-  DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a;
+  DBuf<int> cptr, cent, cact, aptr, aent, apair, p2a, xoff, tcut, tmeta;
   DBuf<short> aent16, cact16;
-  DBuf<double> cscale, t, s, u, rhs, r, p, q, w, tprev, tprev2;
+  DBuf<unsigned> xent;
+  DBuf<double> cscale, t, s, u, rhs, r, p, q, w, tprev, tprev2, tsc, tdinv, xs;
   DBuf<double> ip1, ip2, ip3, ipv, ipP;  // IPM scratch
   DBuf<float> ns32;                      // sign-iteration scratch
 
@@ -288,15 +367,26 @@ struct Slot {
         (rc = aent16.ensure(ne)) || (rc = cact16.ensure(ne)) ||
         (rc = p2a.ensure(nn)))
       return rc;
+    if (!cs.tcut.empty() &&
+        ((rc = xoff.ensure(cs.xoff.size())) || (rc = xent.ensure(cs.xent.size())) ||
+         (rc = tcut.ensure(m)) || (rc = tmeta.ensure(m)) || (rc = tsc.ensure(m)) ||
+         (rc = tdinv.ensure(m)) || (rc = xs.ensure(m))))
+      return rc;
     for (DBuf<double>* b : {&cscale, &t, &s, &u, &rhs, &r, &p, &q, &tprev, &tprev2})
       if ((rc = b->ensure(m))) return rc;
-    return w.ensure(na);
+    // pair sums of the two-pass product, or the cut-side A u of the one-pass one
+    return w.ensure(std::max(na, m));
   }
   void release() {
     for (DBuf<double>* b : {&L, &X, &Q, &W, &V, &S1, &S2, &ev, &evp, &y, &cscale, &t, &s, &u, &rhs,
                             &r, &p, &q, &w, &tprev, &tprev2})
       b->release();
-    for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a}) b->release();
+    for (DBuf<int>* b : {&cptr, &cent, &cact, &aptr, &aent, &apair, &p2a, &xoff, &tcut, &tmeta})
+      b->release();
+    xent.release();
+    tsc.release();
+    tdinv.release();
+    xs.release();
     aent16.release();
     cact16.release();
     for (DBuf<double>* b : {&ip1, &ip2, &ip3, &ipv, &ipP, &gwval, &gwbest}) b->release();
@@ -484,6 +574,15 @@ inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {
   v.aent = sl.aent.p;
   v.aent16 = cs.aent16.empty() ? nullptr : sl.aent16.p;
   v.cact16 = cs.cact16.empty() ? nullptr : sl.cact16.p;
+  const bool xr = !cs.tcut.empty();
+  v.xrows = xr ? 1 : 0;
+  v.xoff = xr ? sl.xoff.p : nullptr;
+  v.xent = xr ? sl.xent.p : nullptr;
+  v.tcut = xr ? sl.tcut.p : nullptr;
+  v.tmeta = xr ? sl.tmeta.p : nullptr;
+  v.tsc = xr ? sl.tsc.p : nullptr;
+  v.tdinv = xr ? sl.tdinv.p : nullptr;
+  v.xs = xr ? sl.xs.p : nullptr;
   v.apair = sl.apair.p;
   v.p2a = sl.p2a.p;
   v.t = sl.t.p;
@@ -516,6 +615,14 @@ inline int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {
       (rc = up(sl.apair.p, cs.apair.data(), cs.apair.size(), s)) ||
       (rc = up(sl.p2a.p, cs.p2a.data(), cs.p2a.size(), s)) ||
       (rc = up(sl.cscale.p, cs.cscale.data(), cs.cscale.size(), s)))
+    return rc;
+  if (!cs.tcut.empty() &&
+      ((rc = up(sl.xoff.p, cs.xoff.data(), cs.xoff.size(), s)) ||
+       (rc = up(sl.xent.p, cs.xent.data(), cs.xent.size(), s)) ||
+       (rc = up(sl.tcut.p, cs.tcut.data(), cs.tcut.size(), s)) ||
+       (rc = up(sl.tmeta.p, cs.tmeta.data(), cs.tmeta.size(), s)) ||
+       (rc = up(sl.tsc.p, cs.tsc.data(), cs.tsc.size(), s)) ||
+       (rc = up(sl.tdinv.p, cs.tdinv.data(), cs.tdinv.size(), s))))
     return rc;
   return MADAM_OK;
 }

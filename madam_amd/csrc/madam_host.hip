@@ -42,6 +42,12 @@ struct TuneCell {
 };
 std::vector<TuneCell> g_tune;
 
+// LDS bytes the cut solve may count on (0: the device's; lower caps move
+// nodes to the kernels with vectors in HBM) and whether its one-pass kernel
+// may serve nodes (madam_set_cg_lds_cap)
+size_t g_cg_lds_cap = 0;
+int g_cg_one_pass = 1;
+
 // The cell of the smallest tabulated n_p >= np (else the largest) and of
 // the largest tabulated node count <= active (else the smallest); 0
 // without a table.
@@ -122,7 +128,8 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
       TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
       TIMED(PH_CG, st,
             madam::launch_cg(E.views.p, hv.data(), k, o.cg_rtol, o.cg_adapt, o.cg_maxit,
-                             E.lds_limit, st));
+                             g_cg_lds_cap ? std::min(g_cg_lds_cap, E.lds_limit) : E.lds_limit,
+                             g_cg_one_pass, st));
       TIMED(PH_W, st, madam::launch_build_w(E.views.p, k, nmax, mmax, 0, st));
       if (o.projection == 1) {
         TIMED(PH_EIG, st, {
@@ -755,6 +762,16 @@ int madam_load_tuning(const char* path) {
 }
 
 int madam_tuned_layout(int n, int active) { return tuned_layout((n + 15) & ~15, active); }
+
+int madam_set_cg_lds_cap(long bytes, int one_pass) {
+  if (bytes < 0) {
+    set_err("madam_set_cg_lds_cap: negative cap %ld", bytes);
+    return MADAM_E_ARG;
+  }
+  g_cg_lds_cap = (size_t)bytes;
+  g_cg_one_pass = one_pass ? 1 : 0;
+  return MADAM_OK;
+}
 
 int madam_time_projection(int n, int nb, int layout, int reps, double* ms) {
   int rc = ensure_open();

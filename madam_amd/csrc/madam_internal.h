@@ -43,11 +43,22 @@ struct NodeView {
   const int *aent;      // aptr[nact]: c or ~c
   const short *aent16;  // aent and cact in 16 bits (null unless m, nact < 2^15)
   const short *cact16;
+  // sliced Gram rows of BB^T for the one-pass cut solve (null when not
+  // built): slot i = cut tcut[i], of thread i % XCUT_THREADS; slice s =
+  // slots 64s..64s+63 with word q of lane l at xent[xoff[s] + 64 q + l]
+  const int *xoff;      // m/64 + 1
+  const unsigned *xent; // words: LDS byte offset 8 (2 c' + neg) of +-ps[c'] (16 m: zero)
+  const int *tcut;      // m: cut of each slot (rows longest first)
+  const int *tmeta;     // m: pair entries of each slot's cut
+  const double *tsc;    // m: scale of each slot's cut
+  const double *tdinv;  // m: diagonal preconditioner of each slot
+  double *xs;           // m: the cut solve's iterate in slot order
   const int *apair;     // nact: lower linear index of the active pair
   const int *p2a;       // n*n: active-pair id of lower index lo, or -1
 
   double *t, *s, *u, *rhs;             // m
-  double *tprev, *tprev2;              // m: the cut solve's solutions one and two iterations back
+  double *tprev, *tprev2;              // m: the cut solve's solutions one and two iterations
+                                       // back (in slot order for the one-pass solve)
   double *cg_r, *cg_p, *cg_q, *cg_w;   // m, m, m, nact
 
   // per-node ADMM scalars
@@ -64,14 +75,19 @@ struct NodeView {
   int ns_fail;    // the latest sign-iteration projection failed its guard
   int ns_fallbacks;  // projections redone by FP64 Jacobi after a failed guard
   int ne;         // cptr[m]: pair entries of all cuts (= aptr[nact])
+  int xrows;      // the sliced Gram rows are built
 };
+
+// threads of the one-pass cut solve (one workgroup per node)
+constexpr int XCUT_THREADS = 1024;
 
 // Launch helpers implemented in madam_device.hip.  All take the device array
 // of nodes and the batch size, and run on `stream`.
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
-// hv: host copies of the views (per-node choice of the cut-solve kernel)
+// hv: host copies of the views (per-node choice of the cut-solve kernel);
+// use_x: the one-pass kernel may serve nodes with expanded rows
 void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
-               size_t lds_limit, hipStream_t stream);
+               size_t lds_limit, int use_x, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
                    int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
