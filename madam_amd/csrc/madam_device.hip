@@ -872,38 +872,96 @@ __global__ __launch_bounds__(WG) void k_cg_x(NodeView* nodes, double rtol, doubl
 // Cut duals (admm.c:557-560, 589-591) and the matrix to project
 // W = L + X/sigma - Diag(y) - B^T(t) (admm.c:564-568); with use_s the safe
 // bound matrix L - Diag(y) - B^T(s) (admm.c:727-729) instead.  Lower
-// triangle computed in the reference's order, mirrored to the upper.
+// triangle computed in the reference's order, mirrored to the upper: an
+// active pair's entry starts from L (+ X/sigma) and subtracts its cuts'
+// terms in cut order, one pair per thread (two at a time); every other
+// lower entry (the column pairs (j, n-1-j) of k_post_part, four at a time)
+// is L (+ X/sigma), minus y on the diagonal.  The two passes write
+// disjoint entries.
 __global__ __launch_bounds__(WGS) void k_build_w(NodeView* nodes, int use_s) {
 #pragma clang fp contract(off)
   NodeView& nd = nodes[blockIdx.y];
   if (!use_s && nd.done) return;
-  const int n = nd.n, m = nd.m;
+  const int n = nd.n, m = nd.m, na = nd.nact;
   const double sg = nd.sigma;
-  const double* __restrict__ vec = use_s ? nd.s : nd.t;
-  if (!use_s)
-    for (int c = blockIdx.x * WGS + threadIdx.x; c < m; c += gridDim.x * WGS) {
-      const double v = nd.t[c] - nd.u[c] / sg;
-      nd.s[c] = v > 0 ? v : 0;
-      nd.u[c] = v < 0 ? -sg * v : 0;
+  const int gt = blockIdx.x * WGS + threadIdx.x, gs = gridDim.x * WGS;
+  const GMEM double* __restrict__ Lm = gmem(nd.L);
+  const GMEM double* __restrict__ Xm = gmem(nd.X);
+  const GMEM double* __restrict__ cs = gmem(nd.cscale);
+  const GMEM int* __restrict__ aptr = gmem(nd.aptr);
+  const GMEM int* __restrict__ apair = gmem(nd.apair);
+  const GMEM int* __restrict__ p2a = gmem(nd.p2a);
+  GMEM double* __restrict__ Wm = gmem(nd.W);
+  const GMEM double* __restrict__ vec = gmem(use_s ? nd.s : nd.t);
+  if (!use_s) {
+    GMEM double* __restrict__ sv = gmem(nd.s);
+    GMEM double* __restrict__ uv = gmem(nd.u);
+    const GMEM double* __restrict__ tv = gmem(nd.t);
+    for (int c = gt; c < m; c += gs) {
+      const double v = tv[c] - uv[c] / sg;
+      sv[c] = v > 0 ? v : 0;
+      uv[c] = v < 0 ? -sg * v : 0;
     }
+  }
   // the dual step above must finish before use_s readers run; callers order
   // them with separate launches.
-  const int total = n * n;
-  for (int f = blockIdx.x * WGS + threadIdx.x; f < total; f += gridDim.x * WGS) {
-    const int i = f % n, j = f / n;
-    if (i < j) continue;
-    const int lo = i + j * n;
-    double w = use_s ? nd.L[lo] : nd.L[lo] + nd.X[lo] / sg;
-    if (i == j) {
-      w -= nd.y[i];
-    } else {
-      const int a = nd.p2a[lo];
-      if (a >= 0)
-        w = signed_sum<true>(nd.aent, nd.aptr[a], nd.aptr[a + 1], w,
-                             [&](int c) { return (0.5 * nd.cscale[c]) * vec[c]; });
+  for (int a0 = gt; a0 < na; a0 += 2 * gs) {
+    int lo[2], e0[2], e1[2];
+    double w[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int a = a0 + u * gs < na ? a0 + u * gs : a0;
+      lo[u] = apair[a];
+      e0[u] = aptr[a];
+      e1[u] = aptr[a + 1];
     }
-    nd.W[lo] = w;
-    nd.W[j + i * n] = w;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) w[u] = use_s ? Lm[lo[u]] : Lm[lo[u]] + Xm[lo[u]] / sg;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (a0 + u * gs >= na) continue;
+      const double wv = signed_sum<true>(nd.aent, e0[u], e1[u], w[u],
+                                         [&](int c) { return (0.5 * cs[c]) * vec[c]; });
+      const int i = lo[u] % n, j = lo[u] / n;
+      Wm[lo[u]] = wv;
+      Wm[j + i * n] = wv;
+    }
+  }
+  const GMEM double* __restrict__ yv = gmem(nd.y);
+  const int np1 = n + 1, ntot = ((n + 1) >> 1) * np1;
+  for (int f0 = gt; f0 < ntot; f0 += 4 * gs) {
+    int lo[4], up[4], act[4];
+    bool ok[4];
+    double l4[4], x4[4], y4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * gs;
+      const int pp = f / np1, r = f - pp * np1;
+      int i, j;
+      if (r < n - pp) {
+        j = pp;
+        i = pp + r;
+      } else {
+        j = n - 1 - pp;
+        i = j + (r - (n - pp));
+      }
+      ok[u] = f < ntot && i < n && (r < n - pp || j != pp);
+      if (!ok[u]) i = j = 0;
+      lo[u] = i + j * n;
+      up[u] = j + i * n;
+      act[u] = i == j ? -2 : p2a[lo[u]];
+      l4[u] = Lm[lo[u]];
+      x4[u] = use_s ? 0.0 : Xm[lo[u]];
+      y4[u] = yv[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ok[u] || act[u] >= 0) continue;
+      double w = use_s ? l4[u] : l4[u] + x4[u] / sg;
+      if (act[u] == -2) w -= y4[u];
+      Wm[lo[u]] = w;
+      Wm[up[u]] = w;
+    }
   }
 }
 
@@ -1176,28 +1234,62 @@ __global__ __launch_bounds__(WGS) void k_post_part(NodeView* nodes, double* part
   const int n = nd.n, m = nd.m, tid = threadIdx.x;
   const int g = blockIdx.x, G = gridDim.x, stride = G * WGS;
   const double sg = nd.sigma;
+  const GMEM double* __restrict__ S1 = gmem(nd.S1);
+  const GMEM double* __restrict__ Wm = gmem(nd.W);
+  const GMEM double* __restrict__ Lm = gmem(nd.L);
+  GMEM double* __restrict__ Xm = gmem(nd.X);
+  GMEM double* __restrict__ Qm = gmem(nd.Q);
   double v[7] = {0, 0, 0, 0, 0, 0, 0};  // |X+ - Xold/sigma|^2, |diag X - 1|^2, <L,X>,
                                         // |max(B(X)-1,0)|^2, |s-t|^2, sum t, sum y
-  for (int f = g * WGS + tid; f < n * n; f += stride) {
-    const int i = f % n, j = f / n;
-    if (i < j) continue;
-    const int lo = i + j * n, up = j + i * n;
-    const double xp = nd.S1[lo];
-    const double xo = nd.X[lo];
-    const double xn = sg * xp;
-    const double dd = xp - xo / sg;
-    nd.X[lo] = xn;
-    nd.X[up] = xn;
-    const double qv = xp - nd.W[lo];
-    nd.Q[lo] = qv;
-    nd.Q[up] = qv;
-    if (i == j) {
-      v[0] += dd * dd;
-      v[1] += (xn - 1.0) * (xn - 1.0);
-      v[2] += nd.L[lo] * xn;
-    } else {
-      v[0] += 2.0 * dd * dd;
-      v[2] += 2.0 * nd.L[lo] * xn;
+  // the lower triangle as ceil(n/2) column pairs (j, n-1-j) of n+1 entries
+  // each (rows j..n-1 of column j, then rows n-1-j..n-1 of column n-1-j), so
+  // that every index is an entry; four entries' operands are loaded before
+  // any store (distinct entries: no store reaches another's loads)
+  const int np1 = n + 1, ntot = ((n + 1) >> 1) * np1;
+  for (int f0 = g * WGS + tid; f0 < ntot; f0 += 4 * stride) {
+    int lo[4], up[4];
+    bool ok[4], dg[4];
+    double xp[4], xo[4], wv[4], lv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * stride;
+      const int pp = f / np1, r = f - pp * np1;
+      int i, j;
+      if (r < n - pp) {
+        j = pp;
+        i = pp + r;
+      } else {
+        j = n - 1 - pp;
+        i = j + (r - (n - pp));
+      }
+      ok[u] = f < ntot && i < n && (r < n - pp || j != pp);
+      if (!ok[u]) i = j = 0;
+      lo[u] = i + j * n;
+      up[u] = j + i * n;
+      dg[u] = i == j;
+      xp[u] = S1[lo[u]];
+      xo[u] = Xm[lo[u]];
+      wv[u] = Wm[lo[u]];
+      lv[u] = Lm[lo[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ok[u]) continue;
+      const double xn = sg * xp[u];
+      const double dd = xp[u] - xo[u] / sg;
+      Xm[lo[u]] = xn;
+      Xm[up[u]] = xn;
+      const double qv = xp[u] - wv[u];
+      Qm[lo[u]] = qv;
+      Qm[up[u]] = qv;
+      if (dg[u]) {
+        v[0] += dd * dd;
+        v[1] += (xn - 1.0) * (xn - 1.0);
+        v[2] += lv[u] * xn;
+      } else {
+        v[0] += 2.0 * dd * dd;
+        v[2] += 2.0 * lv[u] * xn;
+      }
     }
   }
   for (int c = g * WGS + tid; c < m; c += stride) {
@@ -1218,11 +1310,18 @@ __global__ __launch_bounds__(WGS) void k_post_part(NodeView* nodes, double* part
 }
 
 __global__ __launch_bounds__(64) void k_post_fin(NodeView* nodes, const double* part, int G) {
+  __shared__ double pl[64 * 8];
   NodeView& nd = nodes[blockIdx.x];
-  if (nd.done || threadIdx.x != 0) return;
+  if (nd.done) return;
+  // the partials into LDS by all lanes at once, then summed in workgroup
+  // order by lane 0
+  const GMEM double* pg = gmem(part) + (size_t)blockIdx.x * 64 * 8;
+  for (int f = threadIdx.x; f < G * 8; f += 64) pl[f] = pg[f];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int g = 0; g < G; ++g)
-    for (int q = 0; q < 7; ++q) v[q] += part[((size_t)blockIdx.x * 64 + g) * 8 + q];
+    for (int q = 0; q < 7; ++q) v[q] += pl[g * 8 + q];
   const int n = nd.n;
   const double sg = nd.sigma;
   nd.sigma_last = sg;

@@ -22,6 +22,17 @@ __device__ __forceinline__ GMEM T* gmem(T* p) {
   return (GMEM T*)p;
 }
 
+// ok ? p[i] : 0, with the load issued unconditionally (from p[0] when not
+// ok) and its value masked.  A load under a condition compiles to a branch
+// whose block waits for the load before its value is used, so a loop of
+// them pays one memory latency per iteration; this form lets a batch of
+// loads be in flight together.  p must hold at least one element.
+template <class P>
+__device__ __forceinline__ double ld0(P p, size_t i, bool ok) {
+  const double v = p[ok ? i : 0];
+  return ok ? v : 0.0;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

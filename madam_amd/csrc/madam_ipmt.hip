@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, int p, int npa
   if (J.skip || J.fail || (J.stop && *J.stop)) return;
   const int n = J.n, P = (n + IB - 1) / IB;
   if (p > P) return;
-  double* __restrict__ A = J.A;
+  GMEM double* __restrict__ A = gmem(J.A);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int rl = lane & 15, kl = lane >> 4;
   const int j0 = p * IB, jp = j0 - IB;
@@ -151,11 +151,11 @@ __global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, int p, int npa
     if (p == 0) {
       for (int f = tid; f < IB * IB; f += 256) {
         const int c = f / IB, r = f - c * IB;
-        Dl[c * (IB + 1) + r] = (c < nbk && r < nbk && r >= c) ? A[(j0 + r) + (size_t)(j0 + c) * n] : 0.0;
+        Dl[c * (IB + 1) + r] = ld0(A, (j0 + r) + (size_t)(j0 + c) * n, c < nbk && r < nbk && r >= c);
       }
       for (int f = tid; f < IB * RP; f += 256) {
         const int c = f / RP, r = f - c * RP;
-        Rl[c * (RP + 1) + r] = (c < nbk && r < nrow) ? A[(rbeg + r) + (size_t)(j0 + c) * n] : 0.0;
+        Rl[c * (RP + 1) + r] = ld0(A, (rbeg + r) + (size_t)(j0 + c) * n, c < nbk && r < nrow);
       }
     } else {
       // tiles 0..2: the diagonal block (0,0), (1,0), (1,1); then RP/16 row
@@ -178,16 +178,19 @@ __global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, int p, int npa
         for (int v = 0; v < 4; ++v) {
           const int r = r0 + kl + 4 * v, c = c0 + rl;
           const bool ok = c < j0 + nbk && (dg ? (r < j0 + nbk && r >= c) : r < rbeg + nrow);
-          acc[v] = ok ? A[r + (size_t)c * n] : 0.0;
+          acc[v] = ld0(A, r + (size_t)c * n, ok);
         }
         const int ra = r0 + rl, ca = c0 + rl;
+        double av[IB / 4], bv[IB / 4];
 #pragma unroll
         for (int kq = 0; kq < IB / 4; ++kq) {
           const int k = jp + 4 * kq + kl;  // panel p-1 is IB columns wide
-          const double a = ra < n ? -A[ra + (size_t)k * n] : 0.0;
-          const double b = ca < n ? A[ca + (size_t)k * n] : 0.0;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+          av[kq] = ld0(A, ra + (size_t)k * n, ra < n);
+          bv[kq] = ld0(A, ca + (size_t)k * n, ca < n);
         }
+#pragma unroll
+        for (int kq = 0; kq < IB / 4; ++kq)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ra < n ? -av[kq] : 0.0, bv[kq], acc, 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = r0 + kl + 4 * v, c = c0 + rl;
@@ -258,16 +261,19 @@ __global__ __launch_bounds__(256) void k_chol_step(CholJob* jobs, int p, int npa
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = r0 + kl + 4 * v, c = c0 + rl;
-    acc[v] = (r < n && c < n && r >= c) ? A[r + (size_t)c * n] : 0.0;
+    acc[v] = ld0(A, r + (size_t)c * n, r < n && c < n && r >= c);
   }
   const int ra = r0 + rl, ca = c0 + rl;
+  double av[IB / 4], bv[IB / 4];
 #pragma unroll
   for (int kq = 0; kq < IB / 4; ++kq) {
     const int k = jp + 4 * kq + kl;
-    const double a = ra < n ? -A[ra + (size_t)k * n] : 0.0;
-    const double b = ca < n ? A[ca + (size_t)k * n] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    av[kq] = ld0(A, ra + (size_t)k * n, ra < n);
+    bv[kq] = ld0(A, ca + (size_t)k * n, ca < n);
   }
+#pragma unroll
+  for (int kq = 0; kq < IB / 4; ++kq)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ra < n ? -av[kq] : 0.0, bv[kq], acc, 0, 0, 0);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = r0 + kl + 4 * v, c = c0 + rl;
@@ -296,13 +302,14 @@ __global__ __launch_bounds__(256) void k_trtri_step(NodeView* nodes, const IpmCt
   const int j0 = p * IB, nbk = min(IB, n - j0), j1 = j0 + nbk, m2 = n - j1;
   const int x = blockIdx.x;
   if (x > 0 && 16 * (x - 1) >= m2) return;
-  const double* __restrict__ L = nd.ipP;
-  double* __restrict__ W = nd.ip3;
+  const GMEM double* __restrict__ L = gmem(nd.ipP);
+  GMEM double* __restrict__ W = gmem(nd.ip3);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int f = tid; f < IB * IB; f += 256) {
     const int c = f / IB, r = f - c * IB;
-    Ll[c * (IB + 1) + r] =
-        (c < nbk && r < nbk && r >= c) ? L[(j0 + r) + (size_t)(j0 + c) * n] : (r == c ? 1.0 : 0.0);
+    const bool in = c < nbk && r < nbk && r >= c;
+    const double lv = ld0(L, (j0 + r) + (size_t)(j0 + c) * n, in);
+    Ll[c * (IB + 1) + r] = in ? lv : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
   if (wv == 0) {
@@ -345,11 +352,20 @@ __global__ __launch_bounds__(256) void k_trtri_step(NodeView* nodes, const IpmCt
   const int ka = half ? kh : j1, kb = half ? kend : min(kh, kend);
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   const int r = r0 + rl, c = 16 * ct + rl;
-  for (int k0 = ka; k0 < kb; k0 += 4) {
-    const int q = k0 + kl;
-    const double a = (q < kb && r < n && q <= r) ? W[r + (size_t)q * n] : 0.0;
-    const double bq = (q < kb && c < nbk) ? L[q + (size_t)(j0 + c) * n] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq, acc, 0, 0, 0);
+  // eight k steps' operands in flight at a time
+  for (int k0 = ka; k0 < kb; k0 += 32) {
+    double a[8], bq[8];
+#pragma unroll
+    for (int s8 = 0; s8 < 8; ++s8) {
+      const int q = k0 + 4 * s8 + kl;
+      a[s8] = ld0(W, r + (size_t)q * n, q < kb && r < n && q <= r);
+      bq[s8] = ld0(L, q + (size_t)(j0 + c) * n, q < kb && c < nbk);
+    }
+#pragma unroll
+    for (int s8 = 0; s8 < 8; ++s8) {
+      if (k0 + 4 * s8 >= kb) break;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s8], bq[s8], acc, 0, 0, 0);
+    }
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) Pl[wv][kl + 4 * v][rl] = acc[v];
@@ -382,17 +398,28 @@ __global__ __launch_bounds__(256) void k_lauum(NodeView* nodes, const IpmCtl* ct
   tri_tile(t, ti, tj);
   const int i0 = 16 * ti, c0 = 16 * tj;
   const int rl = lane & 15, kl = lane >> 4;
-  const double* __restrict__ W = nd.ip3;
-  double* __restrict__ Zi = nd.S1;
+  const GMEM double* __restrict__ W = gmem(nd.ip3);
+  GMEM double* __restrict__ Zi = gmem(nd.S1);
   const int i = i0 + rl, j = c0 + rl;
   f64x4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-  for (int k0 = i0; k0 < n; k0 += 8) {
+  // four pairs of k steps' operands in flight at a time (chain u takes the
+  // steps k0 + 4 u + 8 s, as before)
+  for (int k0 = i0; k0 < n; k0 += 32) {
+    double a[4][2], bq[4][2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int k = k0 + 4 * u + kl;
-      const double a = (k < n && i < n && k >= i) ? W[k + (size_t)i * n] : 0.0;
-      const double bq = (k < n && j < n && k >= j) ? W[k + (size_t)j * n] : 0.0;
-      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq, acc[u], 0, 0, 0);
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int k = k0 + 8 * s4 + 4 * u + kl;
+        a[s4][u] = ld0(W, k + (size_t)i * n, k < n && i < n && k >= i);
+        bq[s4][u] = ld0(W, k + (size_t)j * n, k < n && j < n && k >= j);
+      }
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      if (k0 + 8 * s4 >= n) break;
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4][u], bq[s4][u], acc[u], 0, 0, 0);
     }
   }
   const f64x4 s = acc[0] + acc[1];
@@ -510,7 +537,7 @@ __global__ __launch_bounds__(256) void k_ipmt_solve(NodeView* nodes, const IpmCt
   if (ctl[b].done) return;
   const NodeView& nd = nodes[b];
   const int n = nd.n, P = (n + IB - 1) / IB;
-  const double* __restrict__ R = nd.ipP;
+  const GMEM double* __restrict__ R = gmem(nd.ipP);
   double* dy = nd.ipv;
   double* dy1 = nd.ipv + n;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -523,7 +550,7 @@ __global__ __launch_bounds__(256) void k_ipmt_solve(NodeView* nodes, const IpmCt
       double v[IB];
 #pragma unroll
       for (int q = 0; q < IB; ++q)
-        v[q] = (r < nbk && q < nbk && q <= r) ? R[(j0 + r) + (size_t)(j0 + q) * n] : 0.0;
+        v[q] = ld0(R, (j0 + r) + (size_t)(j0 + q) * n, r < nbk && q < nbk && q <= r);
       double zr = r < nbk ? z[j0 + r] : 0.0;
 #pragma unroll
       for (int c = 0; c < IB; ++c) {
@@ -539,8 +566,14 @@ __global__ __launch_bounds__(256) void k_ipmt_solve(NodeView* nodes, const IpmCt
     }
     __syncthreads();
     for (int i = j1 + tid; i < n; i += 256) {
+      // the block's IB columns loaded together, subtracted in column order
+      double rv[IB];
+#pragma unroll
+      for (int c = 0; c < IB; ++c) rv[c] = ld0(R, i + (size_t)(j0 + c) * n, c < nbk);
       double s = z[i];
-      for (int c = 0; c < nbk; ++c) s -= R[i + (size_t)(j0 + c) * n] * z[j0 + c];
+#pragma unroll
+      for (int c = 0; c < IB; ++c)
+        if (c < nbk) s -= rv[c] * z[j0 + c];
       z[i] = s;
     }
     __syncthreads();
@@ -550,7 +583,15 @@ __global__ __launch_bounds__(256) void k_ipmt_solve(NodeView* nodes, const IpmCt
     for (int kk = wv; kk < nbk; kk += 4) {
       const int k = j0 + kk;
       double part = 0.0;
-      for (int i = j1 + lane; i < n; i += 64) part += R[i + (size_t)k * n] * z[i];
+      // four rows' loads in flight, added in row order
+      for (int i = j1 + lane; i < n; i += 256) {
+        double rv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rv[u] = ld0(R, (i + 64 * u) + (size_t)k * n, i + 64 * u < n);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i + 64 * u < n) part += rv[u] * z[i + 64 * u];
+      }
       part = wave_sum(part);
       if (lane == 0) sv[kk] = part;
     }
@@ -560,7 +601,7 @@ __global__ __launch_bounds__(256) void k_ipmt_solve(NodeView* nodes, const IpmCt
       double u[IB];  // column r of the diagonal block: u[q] = R(j0 + q, j0 + r)
 #pragma unroll
       for (int q = 0; q < IB; ++q)
-        u[q] = (r < nbk && q < nbk && q >= r) ? R[(j0 + q) + (size_t)(j0 + r) * n] : 0.0;
+        u[q] = ld0(R, (j0 + q) + (size_t)(j0 + r) * n, r < nbk && q < nbk && q >= r);
       double acc = r < nbk ? z[j0 + r] - sv[r] : 0.0;
 #pragma unroll
       for (int i = IB - 1; i >= 0; --i) {
@@ -601,11 +642,11 @@ __global__ __launch_bounds__(256) void k_ipmt_gemm(NodeView* nodes, const IpmCtl
   const int rb = 16 * (wv < 2 ? ti : tj), cbs = 16 * (wv < 2 ? tj : ti);
   const int kh = ((n + 7) >> 3) << 2;
   const int ka = (wv & 1) ? kh : 0, kb = (wv & 1) ? n : min(kh, n);
-  const double* __restrict__ Zi = nd.S1;
-  const double* __restrict__ X = nd.X;
-  const double* __restrict__ dX1 = nd.ip1;
-  const double* __restrict__ dy = nd.ipv;
-  const double* __restrict__ dy1 = nd.ipv + n;
+  const GMEM double* __restrict__ Zi = gmem(nd.S1);
+  const GMEM double* __restrict__ X = gmem(nd.X);
+  const GMEM double* __restrict__ dX1 = gmem(nd.ip1);
+  const GMEM double* __restrict__ dy = gmem(nd.ipv);
+  const GMEM double* __restrict__ dy1 = gmem(nd.ipv + n);
   const int r = rb + rl, c = cbs + rl;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   constexpr int NSB = 8;
@@ -615,13 +656,18 @@ __global__ __launch_bounds__(256) void k_ipmt_gemm(NodeView* nodes, const IpmCtl
     for (int s = 0; s < NSB; ++s) {
       const int k = k0 + 4 * s + kl;
       const bool ok = k < kb;
-      a[s] = (ok && r < n) ? Zi[r + (size_t)k * n] : 0.0;
-      double v = 0.0;
-      if (ok && c < n) {
-        const size_t g = c + (size_t)k * n;
-        v = which == 0 ? -dy1[k] * X[g] : -dy[k] * X[g] + (-dy1[k] * dX1[g]);
+      const bool okc = ok && c < n;
+      a[s] = ld0(Zi, r + (size_t)k * n, ok && r < n);
+      const size_t g = c + (size_t)k * n;
+      const double xg = ld0(X, g, okc), d1 = ld0(dy1, k, okc);
+      double v;
+      if (which == 0) {
+        v = -d1 * xg;
+      } else {
+        const double d0 = ld0(dy, k, okc), e1 = ld0(dX1, g, okc);
+        v = -d0 * xg + (-d1 * e1);
       }
-      bv[s] = v;
+      bv[s] = okc ? v : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < NSB; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv[s], acc, 0, 0, 0);

@@ -67,9 +67,9 @@ __device__ inline void tile_decode(int t, int& i0, int& j0) {
 // NS consecutive k-steps of one quarter: all 2 NS operand loads are issued
 // before the first MFMA (one memory latency per NS steps instead of one per
 // step; the accumulation order is unchanged).
-template <int NS>
-__device__ __forceinline__ void ksteps_f32(const float* __restrict__ A, const float* __restrict__ B,
-                                           int np, float alpha, float beta, int ia, int jb,
+// P: const float* or const GMEM float* (global loads)
+template <int NS, class P>
+__device__ __forceinline__ void ksteps_f32(P A, P B, int np, float alpha, float beta, int ia, int jb,
                                            int k, f32x4& acc) {
   float a[NS], b[NS];
 #pragma unroll
@@ -85,16 +85,23 @@ __device__ __forceinline__ void ksteps_f32(const float* __restrict__ A, const fl
   for (int t = 0; t < NS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[t], acc, 0, 0, 0);
 }
 
-__device__ __forceinline__ f32x4 quarter_f32(const float* __restrict__ A,
-                                             const float* __restrict__ B, int np, float alpha,
-                                             float beta, int i0, int j0, int qi) {
+// (the k steps of a quarter in batches of up to 32, every load of a batch
+// in flight together)
+template <class P>
+__device__ __forceinline__ f32x4 quarter_f32(P A, P B, int np, float alpha, float beta, int i0,
+                                             int j0, int qi) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const int kq = np >> 2;
   const int ia = i0 + r, jb = j0 + r;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   int k = qi * kq + q, rem = kq >> 2;
-  for (; rem >= 16; rem -= 16, k += 64) ksteps_f32<16>(A, B, np, alpha, beta, ia, jb, k, acc);
+  for (; rem >= 32; rem -= 32, k += 128) ksteps_f32<32>(A, B, np, alpha, beta, ia, jb, k, acc);
+  if (rem >= 16) {
+    ksteps_f32<16>(A, B, np, alpha, beta, ia, jb, k, acc);
+    rem -= 16;
+    k += 64;
+  }
   switch (rem) {
 #define MADAM_KS(N) \
   case N:           \
@@ -113,8 +120,8 @@ __device__ __forceinline__ f32x4 quarter_f32(const float* __restrict__ A,
 // only the lower element of each (i,j)/(j,i) pair is stored, to both
 // places: the iterates stay exactly symmetric, which the transposed operand
 // reads rely on (an asymmetric part would grow step by step)
-__device__ __forceinline__ void store_tile_f32(float* __restrict__ Cm, int np, int i0, int j0,
-                                               f32x4 acc) {
+template <class P>
+__device__ __forceinline__ void store_tile_f32(P Cm, int np, int i0, int j0, f32x4 acc) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -166,16 +173,24 @@ __device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const 
 // (k and tiles past n are masked), quarters of kq = 4 ceil(n / 16).  The
 // result register v of lane l holds row (l/16) + 4v, column l%16
 // (measured on gfx950).
-template <int NS>
-__device__ __forceinline__ void ksteps_f64(const double* __restrict__ A,
-                                           const double* __restrict__ B, int n, double alpha,
-                                           double beta, int ia, int jb, int k, f64x4& acc) {
+template <int NS, class P>
+__device__ __forceinline__ void ksteps_f64(P A, P B, int n, double alpha, double beta, int ia,
+                                           int jb, int k, f64x4& acc) {
+  // every load from an in-range address (row, k clamped to n - 1), its value
+  // masked afterwards: no load waits behind a branch
   double a[NS], b[NS];
+  const int iac = ia < n ? ia : n - 1, jbc = jb < n ? jb : n - 1;
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    const int kk = k + 4 * t, kc = kk < n ? kk : n - 1;
+    a[t] = A[iac + (size_t)kc * n];
+    b[t] = B[jbc + (size_t)kc * n];
+  }
 #pragma unroll
   for (int t = 0; t < NS; ++t) {
     const int kk = k + 4 * t;
-    a[t] = (ia < n && kk < n) ? A[ia + (size_t)kk * n] : 0.0;
-    double bb = (jb < n && kk < n) ? beta * B[jb + (size_t)kk * n] : 0.0;
+    a[t] = (ia < n && kk < n) ? a[t] : 0.0;
+    double bb = (jb < n && kk < n) ? beta * b[t] : 0.0;
     if (jb == kk && kk < n) bb += alpha;
     b[t] = bb;
   }
@@ -183,22 +198,24 @@ __device__ __forceinline__ void ksteps_f64(const double* __restrict__ A,
   for (int t = 0; t < NS; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], acc, 0, 0, 0);
 }
 
-__device__ __forceinline__ f64x4 quarter_f64(const double* __restrict__ A,
-                                             const double* __restrict__ B, int n, double alpha,
-                                             double beta, int i0, int j0, int qi) {
+template <class P>
+__device__ __forceinline__ f64x4 quarter_f64(P A, P B, int n, double alpha, double beta, int i0,
+                                             int j0, int qi) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const int kq = ((n + 15) >> 4) << 2;
   const int ia = i0 + r, jb = j0 + r;
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   int k = qi * kq + q, rem = kq >> 2;
-  for (; rem >= 8; rem -= 8, k += 32) ksteps_f64<8>(A, B, n, alpha, beta, ia, jb, k, acc);
+  for (; rem >= 16; rem -= 16, k += 64) ksteps_f64<16>(A, B, n, alpha, beta, ia, jb, k, acc);
   switch (rem) {
 #define MADAM_KS(N) \
   case N:           \
     ksteps_f64<N>(A, B, n, alpha, beta, ia, jb, k, acc); \
     break;
     MADAM_KS(1) MADAM_KS(2) MADAM_KS(3) MADAM_KS(4) MADAM_KS(5) MADAM_KS(6) MADAM_KS(7)
+    MADAM_KS(8) MADAM_KS(9) MADAM_KS(10) MADAM_KS(11) MADAM_KS(12) MADAM_KS(13) MADAM_KS(14)
+    MADAM_KS(15)
 #undef MADAM_KS
     default:
       break;
@@ -206,8 +223,8 @@ __device__ __forceinline__ f64x4 quarter_f64(const double* __restrict__ A,
   return acc;
 }
 
-__device__ __forceinline__ void store_tile_f64(double* __restrict__ Cm, int n, int i0, int j0,
-                                               f64x4 acc) {
+template <class P>
+__device__ __forceinline__ void store_tile_f64(P Cm, int n, int i0, int j0, f64x4 acc) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -412,13 +429,13 @@ __global__ __launch_bounds__(64) void k_ns_norm(NodeView* nodes) {
   NodeView& nd = nodes[blockIdx.y];
   if (nd.done) return;
   const int n = nd.n;
-  const double* __restrict__ Wm = nd.W;
+  const GMEM double* __restrict__ Wm = gmem(nd.W);
   const int nn = n * n;
   double fv = 0.0;
   int f = blockIdx.x * 64 + threadIdx.x;
-  // the terms of each virtual thread in order, their loads issued 32 at a
-  // time (a latency per 32 terms instead of per term)
-  constexpr int NB = 32;
+  // the terms of each virtual thread in order, their loads issued 128 at a
+  // time (a latency per 128 terms instead of per term)
+  constexpr int NB = 128;
   for (; f + (NB - 1) * WG < nn; f += NB * WG) {
     double v[NB];
 #pragma unroll
@@ -444,8 +461,8 @@ __global__ __launch_bounds__(256) void k_ns_fill(NodeView* nodes, int nb, int sp
   for (int i = 0; i < NS_NORM_WAVES; ++i) r += parts[i];
   const double fro = sqrt(r);
   const float inv = fro > 0.0 ? (float)(1.0 / fro) : 0.f;
-  const double* __restrict__ Wm = nd.W;
-  float* Sb = nd.ns32;
+  const GMEM double* __restrict__ Wm = gmem(nd.W);
+  GMEM float* Sb = gmem(nd.ns32);
   for (size_t f = (size_t)part * 256 + threadIdx.x; f < npp; f += (size_t)nparts * 256) {
     const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
     Sb[f] = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
@@ -471,8 +488,9 @@ __global__ __launch_bounds__(256) void k_ns_gemm32(NodeView* nodes, int nb, int 
   int i0, j0;
   tile_decode(t, i0, j0);
   const size_t npp = (size_t)np * np;
-  float* base = nd.ns32;
-  const f32x4 acc = quarter_f32(base + ia * npp, base + ib * npp, np, alpha, beta, i0, j0, w);
+  GMEM float* base = gmem(nd.ns32);
+  const f32x4 acc = quarter_f32((const GMEM float*)(base + ia * npp),
+                                (const GMEM float*)(base + ib * npp), np, alpha, beta, i0, j0, w);
   if (w > 0) part[w - 1][lane] = acc;
   __syncthreads();
   if (w == 0)
@@ -506,8 +524,8 @@ __global__ __launch_bounds__(128) void k_ns_gemm32b(NodeView* nodes, int nb, int
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const size_t npp = (size_t)np * np;
-  const float* __restrict__ A = nd.ns32 + ia * npp;
-  const float* __restrict__ B = nd.ns32 + ib * npp;
+  const GMEM float* __restrict__ A = gmem(nd.ns32) + ia * npp;
+  const GMEM float* __restrict__ B = gmem(nd.ns32) + ib * npp;
   bool vr[2], vc[2];
   int ra[2], cb[2];
 #pragma unroll
@@ -563,7 +581,7 @@ __global__ __launch_bounds__(128) void k_ns_gemm32b(NodeView* nodes, int nb, int
   }
   __syncthreads();
   if (w != 0) return;
-  float* Cm = nd.ns32 + ic * npp;
+  GMEM float* Cm = gmem(nd.ns32) + ic * npp;
 #pragma unroll
   for (int x = 0; x < 4; ++x)
     if (on[x])
@@ -579,13 +597,14 @@ __global__ __launch_bounds__(256) void k_ns_conv(NodeView* nodes, int nb, int sp
   NodeView& nd = nodes[b];
   if (nd.done) return;
   const int n = nd.n, np = (n + 15) & ~15;
-  const float* S = nd.ns32 + (size_t)is * np * np;
+  const GMEM float* S = gmem(nd.ns32) + (size_t)is * np * np;
+  GMEM double* S2 = gmem(nd.S2);
   int bad = 0;
   for (int f = part * 256 + threadIdx.x; f < n * n; f += nparts * 256) {
     const int j = f / n, i = f - j * n;
     const float v = S[i + (size_t)j * np];
     if (!(fabsf(v) <= 1.5f)) bad = 1;
-    nd.S2[f] = (double)v;
+    S2[f] = (double)v;
   }
   if (bad) nd.ns_fail = 1;  // every writer stores the same value
 }
@@ -605,9 +624,9 @@ __global__ __launch_bounds__(256) void k_ns_gemm64(NodeView* nodes, int nb, int 
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i0, j0;
   tile_decode(t, i0, j0);
-  const double* A = which == 2 ? nd.W : nd.S2;
-  const double* B = which == 0 ? nd.S2 : (which == 1 ? nd.S1 : nd.ip1);
-  double* Cm = which == 1 ? nd.ip1 : nd.S1;
+  const GMEM double* A = gmem(which == 2 ? nd.W : nd.S2);
+  const GMEM double* B = gmem(which == 0 ? nd.S2 : (which == 1 ? nd.S1 : nd.ip1));
+  GMEM double* Cm = gmem(which == 1 ? nd.ip1 : nd.S1);
   const double alpha = which == 0 ? 0.0 : (which == 1 ? 1.5 : 0.5);
   const double beta = which == 0 ? 1.0 : (which == 1 ? -0.5 : 0.5);
   const f64x4 acc = quarter_f64(A, B, n, alpha, beta, i0, j0, w);
