@@ -555,6 +555,9 @@ def main():
     ap.add_argument("--ns-layout", type=int, default=0, choices=[0, 1, 2, 3],
                     help="chosen form's sign-iteration layout: 0 automatic, 1 one workgroup "
                          "per node, 2 tile-parallel")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay recurring ADMM check intervals from captured HIP graphs "
+                         "(default: every kernel launched directly, measured faster)")
     ap.add_argument("--timed-form", default="chosen", choices=["chosen", "plain"],
                     help="form run in the warmup, timed and profiled steps (plain: to profile "
                          "the plain FP64 form alone; the line then reports it as value)")
@@ -580,6 +583,7 @@ def main():
     import madam_amd as G
     G.open_device(0)
     lib = G.lib()
+    G.set_graphs(args.graphs)
     wl = WORKLOADS[args.workload](args)
     cfg, step, cpu = wl[:3]
     extra = wl[3] if len(wl) > 3 else {}
@@ -596,6 +600,16 @@ def main():
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
     value = nodes / dt
+    graphs = None
+    if args.graphs:
+        # the same step once more with every kernel launched directly
+        cap, rep = G.graph_stats()
+        G.set_graphs(False)
+        t1 = time.perf_counter()
+        step(args.timed_form)
+        td = time.perf_counter() - t1
+        graphs = {"captures": cap, "replays": rep, "ms_per_step_direct_launches": td * 1e3}
+        log("direct-launch step done")
 
     roofline = None
     if not args.no_profile and args.timed_form == "chosen":
@@ -696,7 +710,8 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32-mfma+f64", "data": "synthetic", "config": cfg, "roofline": roofline,
             "plain_fp64_gpu": plain, "cpu_baseline": cpub,
-            "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN}}
+            "chosen": {"bound": r["bound"], "admm_iters": r["admm_iters"], **CHOSEN},
+            "hip_graphs": graphs}
     if "admm_calls" in r:
         line["chosen"]["admm_calls"] = r["admm_calls"]
     if args.workload == "c5":

@@ -177,6 +177,20 @@ int madam_tuned_layout(int n, int active);
  * solve the same system and the solutions agree to the PCG tolerance. */
 int madam_set_cg_lds_cap(long bytes, int one_pass);
 
+/* Launch mode of the ADMM iterations (no reference counterpart: the
+ * reference runs one CPU loop, admm.c:520-704).  on = 1: the iterations
+ * between two host checks of the stop flags are captured into a HIP graph
+ * the second time the same launches recur (same batch, orders, cut-solve
+ * plan and options) and replayed from it with one launch after that;
+ * on = 0 (default): every kernel is launched directly.  Both issue the
+ * same kernels with the same arguments in the same order (same bits).
+ * While the phase timer is on (madam_profile) kernels are launched
+ * directly.
+ * madam_graph_stats: graphs captured and replays since the library was
+ * loaded.  Host only. */
+int madam_set_graphs(int on);
+int madam_graph_stats(long *captures, long *replays);
+
 /* Coefficients (a_k, b_k, c_k) of the sign iteration's steps
  * x -> a x + b x^3 + c x^5 used by the mixed-precision projection (no
  * reference counterpart): writes min(cap, steps) triples to coef and

@@ -128,6 +128,20 @@ def set_cg_lds_cap(nbytes, one_pass=True):
     _check(lib().madam_set_cg_lds_cap(C.c_long(nbytes), C.c_int(1 if one_pass else 0)))
 
 
+def set_graphs(on=True):
+    """Replay recurring check intervals of the ADMM iterations from a
+    captured HIP graph, or (on=False, the library's default) launch every
+    kernel directly."""
+    _check(lib().madam_set_graphs(C.c_int(1 if on else 0)))
+
+
+def graph_stats():
+    """(graphs captured, graph replays) since the library was loaded."""
+    c, r = C.c_long(0), C.c_long(0)
+    _check(lib().madam_graph_stats(C.byref(c), C.byref(r)))
+    return int(c.value), int(r.value)
+
+
 def _f64(a):
     return np.asfortranarray(a, dtype=np.float64)
 

@@ -25,14 +25,18 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
+    objs = [os.path.join(LIBDIR, src.replace(".hip", ".o")) for src in SOURCES]
+    cmds = [[HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(SOURCES, objs)]
+    if verbose:
+        for cmd in cmds:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    # the translation units compile independently: a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), (os.cpu_count() or 1), 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        for _ in ex.map(lambda c: subprocess.run(c, check=True), cmds):
+            pass
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)

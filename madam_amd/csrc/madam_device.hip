@@ -1444,17 +1444,19 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   k_admm_prep<<<grid, WGS, 0, st>>>(nodes);
 }
 
-void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
-               size_t lds_limit, int use_x, hipStream_t st) {
+CgPlan plan_cg(const NodeView* hv, int nb, size_t lds_limit, int use_x) {
   // per class: is any node in it, and the LDS its largest node asks for
   // (vectors, plus the staged index streams when those fit)
-  size_t need[6] = {0, 0, 0, 0, 0, 0};
-  bool any[6] = {false, false, false, false, false, false};
+  CgPlan P;
+  for (int c = 0; c < 6; ++c) {
+    P.need[c] = 0;
+    P.any[c] = false;
+  }
   for (int b = 0; b < nb; ++b) {
     const NodeView& v = hv[b];
     if (v.done || v.m == 0) continue;
     const int cls = cg_class(v.m, v.nact, use_x ? v.xrows : 0, lds_limit);
-    any[cls] = true;
+    P.any[cls] = true;
     // LDS: the cut vector (and the pair vector where it lives there) and 64
     // reduction doubles, plus for k_cg1 the index streams when they fit
     const size_t base = cls == 2 ? (2 * (size_t)v.m + 66) * sizeof(double)
@@ -1464,8 +1466,15 @@ void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double 
       staged += ((size_t)v.nact + 1) * sizeof(int) +
                 2 * (size_t)v.ne * (v.aent16 ? sizeof(short) : sizeof(int));
     const size_t want = staged <= lds_limit ? staged : base;
-    if (cls < 5 && want > need[cls]) need[cls] = want;
+    if (cls < 5 && want > P.need[cls]) P.need[cls] = want;
   }
+  return P;
+}
+
+void launch_cg_plan(NodeView* nodes, int nb, const CgPlan& P, double rtol, double adapt,
+                    int maxit, size_t lds_limit, int use_x, hipStream_t st) {
+  const size_t* need = P.need;
+  const bool* any = P.any;
   const int ux = use_x ? 1 : 0;
   if (any[0])
     k_cg1<2><<<nb, WG, need[0], st>>>(nodes, rtol, adapt, maxit, lds_limit, need[0], ux);
@@ -1476,6 +1485,12 @@ void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double 
   if (any[3]) k_cg_big<true><<<nb, WG, need[3], st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
   if (any[4]) k_cg_big<false><<<nb, WG, need[4], st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
   if (any[5]) k_cg<<<nb, WG, 0, st>>>(nodes, rtol, adapt, maxit, lds_limit, ux);
+}
+
+void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
+               size_t lds_limit, int use_x, hipStream_t st) {
+  launch_cg_plan(nodes, nb, plan_cg(hv, nb, lds_limit, use_x), rtol, adapt, maxit, lds_limit,
+                 use_x, st);
 }
 
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t st) {

@@ -647,6 +647,8 @@ inline int sync_views_down(int k) {
 int admm_loop(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
 int admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax, int min_done);
 int admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
+// Drop the captured check-interval graph of admm_advance.
+void release_chunk_graph();
 
 // Subproblems on the device (madam_subp.hip): upload_spl puts SP->L in HBM
 // for the current call; device_subproblems builds PP->L of k nodes into

@@ -88,6 +88,16 @@ void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t stream);
 // use_x: the one-pass kernel may serve nodes with expanded rows
 void launch_cg(NodeView* nodes, const NodeView* hv, int nb, double rtol, double adapt, int maxit,
                size_t lds_limit, int use_x, hipStream_t stream);
+// The same in two parts: which cut-solve kernels a batch needs and the LDS
+// each asks for (from the host views), then their launches.  A plan fixes
+// every launch argument, so equal plans issue identical launches.
+struct CgPlan {
+  size_t need[6];
+  bool any[6];
+};
+CgPlan plan_cg(const NodeView* hv, int nb, size_t lds_limit, int use_x);
+void launch_cg_plan(NodeView* nodes, int nb, const CgPlan& plan, double rtol, double adapt,
+                    int maxit, size_t lds_limit, int use_x, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
                    int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
