@@ -191,6 +191,25 @@ int madam_set_cg_lds_cap(long bytes, int one_pass);
 int madam_set_graphs(int on);
 int madam_graph_stats(long *captures, long *replays);
 
+/* Eigen-solver of the plain form (projection 0), which stands in for the
+ * reference's dsyevr calls (operators.c:579, admm.c:731): orders below n use
+ * the one-workgroup cyclic Jacobi kernel (A and the eigenvectors in LDS
+ * where they fit), orders from n on a two-sided block Jacobi method spread
+ * over many workgroups per node (16-wide blocks, 32 x 32 subproblems
+ * diagonalised by one wave each, the block transformations on FP64 MFMA),
+ * warm-started from the previous projection's eigenvectors.  Both stop at
+ * an off-diagonal sum of squares of at most 1e-20 (Jacobi) or 1e-24 (block
+ * Jacobi) of |A|_F^2.  Default 80 (measured crossover on MI355X: the
+ * warm-started ADMM projection at n = 60 takes 0.33 ms by Jacobi and 0.36
+ * by block Jacobi, at n = 100 0.99 and 0.67 ms, at n = 225 20.1 and
+ * 1.7 ms).  Host only; applies to the calls that follow. */
+int madam_set_bj_min_order(int n);
+/* Diagnostics of the block Jacobi solver: out4 (may be null) receives the
+ * counts since the previous call -- eigen-solves, sweeps, subproblem
+ * solves, inner Jacobi sweeps of the subproblems -- which are then reset;
+ * on = 1 keeps counting (one extra host synchronisation per solve). */
+int madam_bj_stats(int on, long *out4);
+
 /* Coefficients (a_k, b_k, c_k) of the sign iteration's steps
  * x -> a x + b x^3 + c x^5 used by the mixed-precision projection (no
  * reference counterpart): writes min(cap, steps) triples to coef and

@@ -135,6 +135,12 @@ def set_graphs(on=True):
     _check(lib().madam_set_graphs(C.c_int(1 if on else 0)))
 
 
+def set_bj_min_order(n):
+    """Plain-form eigen-solver: orders below n by the one-workgroup Jacobi
+    kernel, orders from n on by the many-workgroup block Jacobi solver."""
+    _check(lib().madam_set_bj_min_order(C.c_int(int(n))))
+
+
 def graph_stats():
     """(graphs captured, graph replays) since the library was loaded."""
     c, r = C.c_long(0), C.c_long(0)

@@ -11,7 +11,8 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libmadam_bound.so")
 SOURCES = ["madam_device.hip", "madam_sep.hip", "madam_ipm.hip", "madam_gw.hip", "madam_ns.hip",
            "madam_host.hip", "madam_bab.hip", "madam_mlp.hip",
-           "madam_cert.hip", "madam_noalloc.hip", "madam_subp.hip", "madam_ipmt.hip"]
+           "madam_cert.hip", "madam_noalloc.hip", "madam_subp.hip", "madam_ipmt.hip",
+           "madam_bjac.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

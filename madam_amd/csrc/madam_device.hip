@@ -1005,10 +1005,11 @@ __host__ __device__ inline size_t jac_fixed_doubles(int h) { return 4 * (size_t)
 
 template <bool A_LDS, bool V_LDS>
 __global__ __launch_bounds__(WG) void k_jacobi(NodeView* nodes, int values_only, int max_sweeps,
-                                               int ignore_done, int form_xplus) {
+                                               int ignore_done, int form_xplus, int n_limit) {
   extern __shared__ double smem[];
   NodeView& nd = nodes[blockIdx.x];
   if (!ignore_done && nd.done) return;
+  if (nd.n >= n_limit) return;  // solved by the block Jacobi kernels
   if (form_xplus == 3) {
     // fallback behind the sign-iteration projection: only its failed nodes
     if (!nd.ns_fail) return;
@@ -1509,18 +1510,19 @@ size_t jacobi_lds_bytes(int nmax, bool with_v) {
 }
 
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
-                   int ignore_done, size_t lds_limit, hipStream_t st, int form_xplus) {
+                   int ignore_done, size_t lds_limit, hipStream_t st, int form_xplus,
+                   int n_limit) {
   const int h = (nmax + (nmax & 1)) / 2;
   const size_t fixed = jac_fixed_doubles(h) * sizeof(double);
   if (!values_only && jacobi_lds_bytes(nmax, true) <= lds_limit) {
     k_jacobi<true, true><<<nb, WG, jacobi_lds_bytes(nmax, true), st>>>(
-        nodes, values_only, max_sweeps, ignore_done, form_xplus);
+        nodes, values_only, max_sweeps, ignore_done, form_xplus, n_limit);
   } else if (jacobi_lds_bytes(nmax, false) <= lds_limit) {
     k_jacobi<true, false><<<nb, WG, jacobi_lds_bytes(nmax, false), st>>>(
-        nodes, values_only, max_sweeps, ignore_done, form_xplus);
+        nodes, values_only, max_sweeps, ignore_done, form_xplus, n_limit);
   } else {
     k_jacobi<false, false><<<nb, WG, fixed, st>>>(nodes, values_only, max_sweeps, ignore_done,
-                                                  form_xplus);
+                                                  form_xplus, n_limit);
   }
 }
 

@@ -649,6 +649,18 @@ int admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int m
 int admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, int mmax);
 // Drop the captured check-interval graph of admm_advance.
 void release_chunk_graph();
+// FP64 eigen-solver of the plain form for orders from madam_set_bj_min_order
+// on (madam_bjac.hip): two-sided block Jacobi over many workgroups per node,
+// with the results the one-workgroup Jacobi kernel leaves (values_only: ev
+// and off_last only; otherwise also evp, M, V and X_+ in S1).
+// Nodes of order bj_min_order() or more go to bj_eig (it skips the others),
+// smaller ones to k_jacobi (launch_jacobi with n_limit = bj_min_order()):
+// each node's solver depends on its own order only.
+bool bj_wanted(int nmax);
+int bj_min_order();
+int bj_eig(NodeView* d_nodes, const NodeView* hv, int nb, int values_only, int ignore_done,
+           int max_sweeps, hipStream_t st);
+void release_bj();
 
 // Subproblems on the device (madam_subp.hip): upload_spl puts SP->L in HBM
 // for the current call; device_subproblems builds PP->L of k nodes into

@@ -163,7 +163,12 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
     if (layout == 0)
       layout = (small ? active <= small_tiles : active < (3 * E.cu_count) / 4) ? 2 : 1;
     const madam::CgPlan plan = madam::plan_cg(hv.data(), k, cg_lds, g_cg_one_pass);
-    auto issue = [&]() {
+    // the block Jacobi solver of the plain form reads its stop flags on the
+    // host between sweeps (no graph capture then)
+    const bool plain_bj = o.projection != 1 && bj_wanted(nmax);
+    int nmin = nmax;
+    for (int b = 0; b < k; ++b) nmin = std::min(nmin, hv[b].n);
+    auto issue = [&]() -> int {
       for (int it = 0; it < chunk; ++it) {
         TIMED(PH_PREP, st, madam::launch_admm_prep(E.views.p, k, mmax, st));
         TIMED(PH_CG, st,
@@ -178,6 +183,14 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
               madam::launch_ns_tiles(E.views.p, k, nmax, st, layout == 3);
             madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 3);
           });
+        } else if (plain_bj) {
+          TIMED(PH_EIG, st, {
+            rc = bj_eig(E.views.p, hv.data(), k, 0, 0, o.jacobi_sweeps, st);
+            if (!rc && nmin < bj_min_order())
+              madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st, 1,
+                                   bj_min_order());
+          });
+          if (rc) return rc;
         } else {
           TIMED(PH_EIG, st,
                 madam::launch_jacobi(E.views.p, k, nmax, 0, o.jacobi_sweeps, 0, E.lds_limit, st,
@@ -185,9 +198,10 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
         }
         TIMED(PH_POST, st, madam::launch_admm_post(E.views.p, k, nmax, mmax, E.postpart.p, st));
       }
+      return MADAM_OK;
     };
     bool graphed = false;
-    if (g_use_graphs && !E.timer.on) {
+    if (g_use_graphs && !E.timer.on && !plain_bj) {
       std::vector<long long> sig = {k, nmax, mmax, layout, chunk, o.projection,
                                     bits_of(o.cg_rtol), bits_of(o.cg_adapt), o.cg_maxit,
                                     o.jacobi_sweeps, (long long)(uintptr_t)E.views.p,
@@ -205,8 +219,8 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
         hipGraph_t graph = nullptr;
         bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
-          issue();
-          ok = hipStreamEndCapture(st, &graph) == hipSuccess && graph != nullptr;
+          ok = issue() == MADAM_OK;
+          ok = hipStreamEndCapture(st, &graph) == hipSuccess && graph != nullptr && ok;
         }
         if (ok) {
           hipGraphExecUpdateResult upd;
@@ -236,7 +250,7 @@ int mh::admm_advance(std::vector<NodeView>& hv, const madam_opts& o, int nmax, i
       }
       g_chunk.seen = sig;
     }
-    if (!graphed) issue();
+    if (!graphed && (rc = issue())) return rc;
     HIP_TRY(hipGetLastError());
     if ((rc = down(hv.data(), E.views.p, k, st))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
@@ -273,8 +287,14 @@ int mh::admm_finish(std::vector<NodeView>& hv, const madam_opts& o, int nmax, in
       madam::launch_safe_hh(E.fviews.p, k, nmax, st);
     else if (o.projection == 1)
       rc = safe_tiles(E.fviews.p, hv.data(), k, nmax, st);
-    else
-      madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0);
+    else {
+      int nmin = nmax;
+      for (int b = 0; b < k; ++b) nmin = std::min(nmin, hv[b].n);
+      if (bj_wanted(nmax)) rc = bj_eig(E.fviews.p, hv.data(), k, 1, 1, 4 * o.jacobi_sweeps, st);
+      if (!rc && nmin < bj_min_order())
+        madam::launch_jacobi(E.fviews.p, k, nmax, 1, 4 * o.jacobi_sweeps, 1, E.lds_limit, st, 0,
+                             bj_min_order());
+    }
     if (!rc) madam::launch_safe_bound(E.fviews.p, k, st);
   });
   if (rc) return rc;
@@ -360,6 +380,7 @@ void madam_close(void) {
   release_ipm_tiles();
   release_safe_tiles();
   release_chunk_graph();
+  release_bj();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   delete g_eng;
@@ -529,7 +550,11 @@ int madam_project_sdp(int n, const double* W, double* X, double* Q, double* Z, i
   const hipStream_t st = g_eng->stream;
   const size_t nn = (size_t)n * n;
   if ((rc = up(sl.W.p, W, nn, st))) return rc;
-  madam::launch_jacobi(g_eng->views.p, 1, n, 0, 60, 1, g_eng->lds_limit, st, 1);
+  if (bj_wanted(n)) {
+    if ((rc = bj_eig(g_eng->views.p, g_eng->hviews.data(), 1, 0, 1, 60, st))) return rc;
+  } else {
+    madam::launch_jacobi(g_eng->views.p, 1, n, 0, 60, 1, g_eng->lds_limit, st, 1);
+  }
   madam::launch_project(g_eng->views.p, 1, n, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;
@@ -578,7 +603,11 @@ int madam_lambda_max(int n, const double* A, double* lam) {
   if ((rc = up(sl.W.p, A, (size_t)n * n, st))) return rc;
   // safe bound with y = 0 and no cuts: f = n * lambda_max
   HIP_TRY(hipMemsetAsync(sl.y.p, 0, n * sizeof(double), st));
-  madam::launch_jacobi(g_eng->views.p, 1, n, 1, 160, 1, g_eng->lds_limit, st, 0);
+  if (bj_wanted(n)) {
+    if ((rc = bj_eig(g_eng->views.p, g_eng->hviews.data(), 1, 1, 1, 160, st))) return rc;
+  } else {
+    madam::launch_jacobi(g_eng->views.p, 1, n, 1, 160, 1, g_eng->lds_limit, st, 0);
+  }
   madam::launch_safe_bound(g_eng->views.p, 1, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_views_down(1))) return rc;

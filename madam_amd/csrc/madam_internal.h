@@ -99,8 +99,11 @@ CgPlan plan_cg(const NodeView* hv, int nb, size_t lds_limit, int use_x);
 void launch_cg_plan(NodeView* nodes, int nb, const CgPlan& plan, double rtol, double adapt,
                     int maxit, size_t lds_limit, int use_x, hipStream_t stream);
 void launch_build_w(NodeView* nodes, int nb, int nmax, int mmax, int use_s, hipStream_t stream);
+// n_limit: nodes of that order or more are skipped (the block Jacobi
+// solver's)
 void launch_jacobi(NodeView* nodes, int nb, int nmax, int values_only, int max_sweeps,
-                   int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus);
+                   int ignore_done, size_t lds_limit, hipStream_t stream, int form_xplus,
+                   int n_limit = 1 << 30);
 size_t jacobi_lds_bytes(int nmax, bool with_v);
 // part: nb x 64 x 8 doubles of scratch for the per-workgroup partial sums
 void launch_admm_post(NodeView* nodes, int nb, int nmax, int mmax, double* part,
