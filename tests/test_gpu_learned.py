@@ -84,11 +84,10 @@ def test_training_lowers_loss():
 
 
 def test_trained_network_gain_over_cold_start():
-    """Train on 150 graphs of the C4 family (G(100, 0.5), w in {-1,0,1}),
-    warm-start 24 fresh graphs: report the ADMM iterations saved against
-    the cold start and against an untrained network, and check that every
-    warm-started bound equals the cold-started one to the ADMM tolerance
-    (the bound is certified whatever the start)."""
+    """Train on 150 graphs of G(100, 0.5), w in {-1,0,1}, warm-start 24 fresh
+    graphs: fewer ADMM iterations in total than the cold start and than an
+    untrained network, and every warm-started bound equal to the cold-started
+    one to the ADMM tolerance (the bound is certified whatever the start)."""
     Ltr = [graph(10000 + s) for s in range(150)]
     Lev = [graph(20000 + s) for s in range(24)]
     Xs = [r.X for r in LW.solve_basic(Ltr, tol=1e-6)]
@@ -102,5 +101,8 @@ def test_trained_network_gain_over_cold_start():
           f"(saved {100 * (1 - nw / nc):.1f}% vs cold, {100 * (1 - nw / nr):.1f}% vs untrained)")
     for c, w in zip(cold, warm):
         assert abs(c.f - w.f) <= 1e-4 * abs(c.f)
-    assert nw <= nr
-    assert nw <= 1.02 * nc
+    # the trained network saves iterations over both (measured 7276 against
+    # 7379 cold and 7373 untrained: 1.4%); tests/test_warm_start_sensitivity.py
+    # shows why no predictor of this accuracy can save much more
+    assert nw < nc
+    assert nw < nr
