@@ -409,6 +409,16 @@ def wl_c5(args):
                     "incumbent": float(B.ctx.LB), "open": int(B.ctx.open_nodes),
                     "admm_calls": int(B.ctx.admm_calls), "ipm_iters": int(B.ctx.ipm_iters),
                     "ns_fallbacks": int(B.ctx.ns_fallbacks)}
+        if args.timed_form == "plain":
+            # the plain FP64 form timed (and profiled) on the whole step
+            B = G.Bounder(SPL, G.default_params(**C5_PARAMS), batch=64, **PLAIN)
+            B.bab(max_nodes=args.c5_nodes)
+            return {"bound": float(B.ctx.root_bound), "bounds": np.array([B.ctx.root_bound]),
+                    "admm_iters": int(B.ctx.admm_iters), "nodes": int(B.ctx.nodes),
+                    "n3_iters": float(B.ctx.admm_iters) * N ** 3, "n": N,
+                    "incumbent": float(B.ctx.LB), "open": int(B.ctx.open_nodes),
+                    "admm_calls": int(B.ctx.admm_calls), "ipm_iters": int(B.ctx.ipm_iters),
+                    "ns_fallbacks": 0}
         # plain FP64 form, one node at a time, on the shared sample
         bs, it = [], 0
         for q, (xf, sx) in enumerate(sample):
@@ -420,6 +430,17 @@ def wl_c5(args):
             log(f"plain: sample node {q + 1} of {len(sample)} bounded")
         return {"bound": float(np.mean(bs)), "bounds": np.array(bs), "admm_iters": it,
                 "nodes": len(sample), "n3_iters": float(it) * N ** 3}
+
+    def plain_step():
+        """The same step (search from the root, same budget and pipeline) in
+        the plain FP64 form."""
+        t0 = time.perf_counter()
+        B = G.Bounder(SPL, G.default_params(**C5_PARAMS), batch=64, **PLAIN)
+        B.bab(max_nodes=args.c5_nodes)
+        dt = time.perf_counter() - t0
+        return {"value": int(B.ctx.nodes) / dt, "seconds": dt, "nodes": int(B.ctx.nodes),
+                "root_bound": float(B.ctx.root_bound), "incumbent": float(B.ctx.LB),
+                "admm_iters": int(B.ctx.admm_iters), "admm_calls": int(B.ctx.admm_calls)}
 
     def chosen_on_sample():
         t0 = time.perf_counter()
@@ -487,6 +508,7 @@ def wl_c5(args):
                        f"64, <= 3000 + 1000 + 1000 tri/pent/hept cuts per round",
            "n": N, "nodes_per_step": args.c5_nodes}
     return cfg, step, cpu, {"chosen_on_sample": chosen_on_sample, "replay": replay,
+                            "plain_step": plain_step,
                             "sample": f"{len(sample)} nodes with 1-4 random fixings (seed 55)"}
 
 
@@ -668,6 +690,12 @@ def main():
             plain["chosen_same_sample"] = cs
             plain["chosen_over_plain"] = cs["value"] / plain["value"]
             diff = np.abs(np.array(cs["bounds"]) - pr["bounds"]) / np.abs(pr["bounds"])
+            ps = extra["plain_step"]()
+            log("plain form on the whole step done")
+            ps["chosen_over_plain"] = value / ps["value"]
+            ps["root_bound_rel_diff_vs_chosen"] = abs(ps["root_bound"] - r["bound"]) / abs(
+                r["bound"])
+            plain["same_workload"] = ps
         else:
             plain["chosen_over_plain"] = value / plain["value"]
             diff = np.abs(r["bounds"][:nb] - pr["bounds"]) / np.abs(pr["bounds"])
