@@ -37,6 +37,8 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int BJ_B = 16;            // block width
 constexpr int BJ_S = 2 * BJ_B;      // subproblem order
 constexpr int BJ_LD = BJ_S + 1;     // LDS row stride of a subproblem
+// stop-test partial sums of one node: nA + npair for nbk <= 64 (n <= 1024)
+constexpr int BJ_PART_MAX = 32 * 31 / 2 + 32;
 
 struct BjJob {
   double* A;     // n_p x n_p, ld n_p
@@ -467,6 +469,7 @@ __global__ __launch_bounds__(256) void k_bj_update(BjJob* jobs, int round) {
 // End of a sweep: the off-diagonal sum of squares in block-pair order and
 // the diagonal's, the stop test off2 <= 1e-24 |A|_F^2.
 __global__ __launch_bounds__(64) void k_bj_check(BjJob* jobs) {
+  __shared__ double pl[BJ_PART_MAX];
   BjJob& J = jobs[blockIdx.x];
   if (J.done) return;
   const int lane = threadIdx.x, np = J.np;
@@ -477,10 +480,15 @@ __global__ __launch_bounds__(64) void k_bj_check(BjJob* jobs) {
     d2 += d * d;
   }
   d2 = wave_sum(d2);
+  // the partial sums into LDS by all lanes at once, then added in order by
+  // lane 0
+  for (int x = lane; x < nA; x += 64) pl[x] = J.part[x];
+  for (int x = lane; x < npair; x += 64) pl[nA + x] = J.part2[x];
+  __syncthreads();
   if (lane != 0) return;
   double off2 = 0.0;
-  for (int x = 0; x < nA; ++x) off2 += J.part[x];
-  for (int x = 0; x < npair; ++x) off2 += J.part2[x];
+  for (int x = 0; x < nA; ++x) off2 += pl[x];
+  for (int x = 0; x < npair; ++x) off2 += pl[nA + x];
   J.off2 = off2;
   J.fro2 = d2 + off2;
   J.sweeps += 1;
@@ -572,6 +580,10 @@ int bj_eig(NodeView* d_nodes, const NodeView* hv, int nb, int values_only, int i
   int nbk_max = 2, ntile_max = 1, upd_max = 1;
   for (int b = 0; b < nb; ++b) {
     const int n = std::max(1, hv[b].n);
+    if (n > MADAM_NMAX) {
+      set_err("bj_eig: order %d above %d", n, MADAM_NMAX);
+      return MADAM_E_ARG;
+    }
     int nbk = (n + BJ_B - 1) / BJ_B;
     nbk += nbk & 1;
     const int np = nbk * BJ_B, npair = nbk / 2, nA = npair * (npair - 1) / 2;
