@@ -169,6 +169,80 @@ __device__ __forceinline__ void sym_gemm_f32(const float* __restrict__ A, const 
   for (int t = wave; t < ntile; t += nw) gemm_tile_f32(A, B, Cm, np, alpha, beta, t);
 }
 
+// The same product over 2 x 2 blocks of tiles: the calling wave computes
+// the (up to four) lower tiles of block t of the ceil(n_p / 32)-square block
+// grid, all four K quarters each, so that every operand value loaded feeds
+// two MFMAs.  Each tile's quarters run the k steps of gemm_tile_f32 in the
+// same order and are summed as ((q0 + q1) + q2) + q3: the same bits.  Used
+// where an operand comes from L2/HBM (256 nodes of order 150: 4.4 against
+// 4.6 us per node and projection; order 170: 5.9 against 6.0); with all
+// iterates in LDS single tiles are faster (order 100: 1.3 against 1.6).
+template <class PA, class PB, class PC>
+__device__ __forceinline__ void gemm_block2_f32(PA A, PB B, PC Cm, int np, float alpha, float beta,
+                                                int t) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const int nt = np >> 4;
+  int bi, bj;
+  tile_decode(t, bi, bj);
+  bi >>= 4;
+  bj >>= 4;
+  bool vr[2], vc[2];
+  int ra[2], cb[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    vr[u] = 2 * bi + u < nt;
+    vc[u] = 2 * bj + u < nt;
+    ra[u] = (vr[u] ? (2 * bi + u) * 16 : 0) + r;  // a missing tile row reads row 0, unused
+    cb[u] = (vc[u] ? (2 * bj + u) * 16 : 0) + r;
+  }
+  bool on[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    on[x] = vr[x >> 1] && vc[x & 1] && 2 * bj + (x & 1) <= 2 * bi + (x >> 1);
+  const int kq = np >> 2;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) acc[u][x] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < kq; s += 4) {
+    float a[4][2], b[4][2];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = u * kq + s + q;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        a[u][v] = A[ra[v] + (size_t)k * np];
+        float bb = beta * B[cb[v] + (size_t)k * np];
+        if (cb[v] == k) bb += alpha;
+        b[u][v] = bb;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        if (on[x])
+          acc[u][x] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][x >> 1], b[u][x & 1], acc[u][x], 0,
+                                                           0, 0);
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    if (on[x])
+      store_tile_f32(Cm, np, (2 * bi + (x >> 1)) * 16, (2 * bj + (x & 1)) * 16,
+                     ((acc[0][x] + acc[1][x]) + acc[2][x]) + acc[3][x]);
+}
+
+template <class PA, class PB, class PC>
+__device__ __forceinline__ void sym_gemm_f32_b2(PA A, PB B, PC Cm, int np, float alpha,
+                                                float beta) {
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int ng = ((np >> 4) + 1) >> 1;
+  const int nblk = ng * (ng + 1) / 2;
+  for (int t = wave; t < nblk; t += nw) gemm_block2_f32(A, B, Cm, np, alpha, beta, t);
+}
+
 // Same in FP64 with v_mfma_f64_16x16x4f64, on n x n matrices of any n
 // (k and tiles past n are masked), quarters of kq = 4 ceil(n / 16).  The
 // result register v of lane l holds row (l/16) + 4v, column l%16
@@ -318,21 +392,31 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   for (int s = 0; s < NS_STEPS; ++s) {
     const float a = NS_COEF[s][0], b = NS_COEF[s][1], c = NS_COEF[s][2];
     if (MODE == 2) {
-      sym_gemm_f32(Lb, Lb, T, np, 0.f, 1.f);  // T = S^2 (S staged)
+      sym_gemm_f32_b2(Lb, Lb, T, np, 0.f, 1.f);  // T = S^2 (S staged)
       __syncthreads();
       stage(T);
-      sym_gemm_f32(Lb, Lb, U, np, b, c);      // U = T (b I + c T)
+      sym_gemm_f32_b2(Lb, Lb, U, np, b, c);      // U = T (b I + c T)
       __syncthreads();
       stage(U);
-      sym_gemm_f32(S, Lb, T, np, a, 1.f);     // S' = S (a I + U), into T
+      sym_gemm_f32_b2(S, Lb, T, np, a, 1.f);     // S' = S (a I + U), into T
       __syncthreads();
       stage(T);
     } else {
-      sym_gemm_f32(S, S, T, np, 0.f, 1.f);  // T = S^2
-      __syncthreads();
-      sym_gemm_f32(T, T, U, np, b, c);      // U = T (b I + c T)
-      __syncthreads();
-      sym_gemm_f32(S, U, T, np, a, 1.f);    // S' = S (a I + U), into T
+      // (all in LDS, MODE 1: single tiles, measured faster there; MODE 0:
+      // 2 x 2 blocks)
+      if (MODE == 1) {
+        sym_gemm_f32(S, S, T, np, 0.f, 1.f);  // T = S^2
+        __syncthreads();
+        sym_gemm_f32(T, T, U, np, b, c);      // U = T (b I + c T)
+        __syncthreads();
+        sym_gemm_f32(S, U, T, np, a, 1.f);    // S' = S (a I + U), into T
+      } else {
+        sym_gemm_f32_b2(S, S, T, np, 0.f, 1.f);
+        __syncthreads();
+        sym_gemm_f32_b2(T, T, U, np, b, c);
+        __syncthreads();
+        sym_gemm_f32_b2(S, U, T, np, a, 1.f);
+      }
       __syncthreads();
     }
     float* tmp = S;
