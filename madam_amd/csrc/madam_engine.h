@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -274,24 +275,27 @@ inline int build_cutset(int n, int ntri, const madam_tri* tri, int npent, const 
   return MADAM_OK;
 }
 
-// f(i) for i in [0, n) on up to 8 host threads (static chunks); the
+// f(i) for i in [0, n) on up to 16 host threads; the
 // per-node host work of a batch (cut structures, separation lists) is
 // independent across nodes.  f must not touch shared state.
 template <class F>
 inline void parallel_for(int n, F f) {
+  // up to 16 threads (one GPU's share of the host), items handed out one at
+  // a time: per-node host work varies with the node's cut counts
   const unsigned hw = std::thread::hardware_concurrency();
-  const int T = std::min<int>(std::min<unsigned>(hw ? hw : 1, 8u), n / 8);
+  const int T = std::min<int>(std::min<unsigned>(hw ? hw : 1, 16u), n / 4);
   if (T <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
+  std::atomic<int> next(0);
   std::vector<std::thread> pool;
   pool.reserve(T - 1);
-  auto run = [&](int t) {
-    for (int i = (int)((long)n * t / T); i < (int)((long)n * (t + 1) / T); ++i) f(i);
+  auto run = [&]() {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
   };
-  for (int t = 1; t < T; ++t) pool.emplace_back(run, t);
-  run(0);
+  for (int t = 1; t < T; ++t) pool.emplace_back(run);
+  run();
   for (auto& th : pool) th.join();
 }
 
