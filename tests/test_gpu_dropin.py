@@ -13,8 +13,18 @@ ranks fail on any tree that branches; the workers share the one GPU).
       madam_amd/dropin/madam_admm.c in place of admm.c and ipm_mc_pk.c:
       ADMM_solver (admm.h:212) and ipm_mc_pk (admm.h:269) run on the GPU
       with the reference's prototypes and globals.
+  oracle/_ref/madam_dropin_ops   as madam_dropin_admm, with
+      madam_amd/dropin/madam_operators.c and madam_evaluate.c in place of
+      operators.c, qap_simulated_annealing.c and evaluate.c: op_B, op_Bt,
+      project_SDP, the separation's annealing (qap_simulated_annealing,
+      admm.h:286) and createSubproblem/getFixedValue (admm.h:246-247) run
+      through the library too.
+  oracle/_ref/madam_ops_check    tests/dropin_ops_check.c: calls op_B,
+      op_Bt, project_SDP, diag, Diag and op_Diag of madam_operators.c
+      through the reference's prototypes and globals (admm.h:272-277,
+      global_var.h); checked against the oracle below.
 
-Both are built by `make -C oracle all` where the reference tree exists.
+All are built by `make -C oracle all` where the reference tree exists.
 Instances: g05_60.0 (rudy, params file values; its root prunes: optimum
 536) and G(60, 0.5) with weights uniform on +-{1..10}, seed 6 (its tree
 branches: the oracle's own B&B needs 7 nodes).  The proven optimum must
@@ -26,6 +36,7 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from oracle import oracle_py as O
@@ -34,7 +45,8 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BINS = {name: os.path.join(ROOT, "oracle", "_ref", name)
-        for name in ("madam_dropin", "madam_dropin_admm")}
+        for name in ("madam_dropin", "madam_dropin_admm", "madam_dropin_ops")}
+OPS_CHECK = os.path.join(ROOT, "oracle", "_ref", "madam_ops_check")
 MPIEXEC = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
 FIELDS = ["sigma0", "scaleSigma", "ADMM_itermax", "tol0", "scaleTol", "minTol", "triag_iter",
           "pent_iter", "hept_iter", "max_outer_iter", "extra_iter", "violated_TriIneq", "TriIneq",
@@ -104,3 +116,105 @@ def test_reference_bab_with_gpu_bounding(tmp_path, instances, binary, inst, proj
     if oracle_nodes > 1:
         # the tree branched: the workers (rank != 0) bounded nodes
         assert nodes > 1
+
+
+def _cut_case(n, seed, nt=40, np_=15, nh=10):
+    """A full (non-symmetric) X -- the operators read its lower triangle --
+    with cuts of every kind (types 1-4, 1-3, 1-4 as the reference's) and t."""
+    rng = np.random.default_rng(seed)
+    X = np.asfortranarray(rng.standard_normal((n, n)) * 0.7)
+    tri = []
+    for _ in range(nt):
+        i, j, k = sorted(rng.choice(n, 3, replace=False), reverse=True)
+        tri.append((int(rng.integers(1, 5)), int(i), int(j), int(k)))
+    pent = [(int(rng.integers(1, 4)), tuple(int(v) for v in rng.permutation(n)[:5]))
+            for _ in range(np_)]
+    hept = [(int(rng.integers(1, 5)), tuple(int(v) for v in rng.permutation(n)[:7]))
+            for _ in range(nh)]
+    t = rng.standard_normal(nt + np_ + nh)
+    return X, tri, pent, hept, t
+
+
+def _write_case(path, X, tri, pent, hept, t):
+    n = X.shape[0]
+    with open(path, "wb") as f:
+        f.write(np.array([n, len(tri), len(pent), len(hept)], np.int32).tobytes())
+        f.write(np.asarray(X, np.float64).tobytes(order="F"))
+        for ty, i, j, k in tri:
+            f.write(np.array([i, j, k, ty], np.int32).tobytes())
+        for ty, perm in pent:
+            f.write(np.array([ty, *perm], np.int32).tobytes())
+        for ty, perm in hept:
+            f.write(np.array([ty, *perm], np.int32).tobytes())
+        f.write(np.asarray(t, np.float64).tobytes())
+
+
+class _Reader:
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        self.o = 0
+
+    def take(self, dtype, count):
+        a = np.frombuffer(self.b, dtype, count, self.o)
+        self.o += a.nbytes
+        return a
+
+    def mat(self, n, m=None):
+        return self.take(np.float64, n * (n if m is None else m)).reshape(
+            (n if m is None else m, n)).T
+
+
+@pytest.mark.skipif(not os.path.exists(OPS_CHECK), reason="oracle/_ref/madam_ops_check not built")
+@pytest.mark.parametrize("n,seed", [(7, 1), (60, 2), (131, 3), (200, 4)])
+def test_reference_operator_prototypes(tmp_path, n, seed):
+    """op_B / op_Bt / project_SDP / diag / Diag / op_Diag with the
+    reference's prototypes and side effects (operators.c:32-619):
+      op_B          X's lower triangle mirrored to its upper; y bit-identical
+                    to the oracle's B(X)
+      op_Bt         X - B^T(t), to 1e-13 (the oracle sums B^T(t) from 0)
+      project_SDP   X's lower triangle = X_+ (eigh of the lower triangle, to
+                    1e-10 max|X|), its upper triangle untouched; Q = X_out -
+                    X_in with a zero upper triangle; M = number of
+                    eigenvalues above 1e-8, W = those eigenvalues ascending
+                    (1e-10), Z Z^T = X_+ (1e-10)
+      diag, Diag, op_Diag  exact"""
+    X0, tri, pent, hept, t = _cut_case(n, seed)
+    _write_case(tmp_path / "in.bin", X0, tri, pent, hept, t)
+    print("madam_ops_check", n, seed, flush=True)
+    r = subprocess.run([OPS_CHECK, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    m = len(tri) + len(pent) + len(hept)
+    R = _Reader(tmp_path / "out.bin")
+    Xs = np.tril(X0) + np.tril(X0, -1).T
+    tri5 = [(ty, i, j, k, 0.0) for ty, i, j, k in tri]
+    y = R.take(np.float64, m)
+    assert np.array_equal(y, O.op_B(Xs, tri5, pent, hept))
+    assert np.array_equal(R.mat(n), Xs)
+    XmB = R.mat(n)
+    assert np.abs(XmB - (X0 - O.op_Bt(n, t, tri5, pent, hept))).max() <= 1e-13
+
+    Xo, Qo = R.mat(n), R.mat(n)
+    M = int(R.take(np.int32, 1)[0])
+    Wv = R.take(np.float64, M)
+    Z = R.mat(n, M) if M else np.zeros((n, 0))
+    w, V = np.linalg.eigh(Xs)
+    Xp = (V * np.where(w > 1e-8, w, 0.0)) @ V.T
+    scale = np.abs(Xs).max()
+    low = np.tril_indices(n)
+    up = np.triu_indices(n, 1)
+    assert np.abs(Xo[low] - Xp[low]).max() <= 1e-10 * scale
+    assert np.array_equal(Xo[up], X0[up])
+    assert np.abs(Qo[low] - (Xp - Xs)[low]).max() <= 1e-10 * scale
+    assert not Qo[up].any()
+    assert M == int((w > 1e-8).sum())
+    assert np.all(np.diff(Wv) >= 0)
+    assert np.abs(Wv - w[w > 1e-8]).max() <= 1e-10 * scale
+    assert np.abs(Z @ Z.T - Xp).max() <= 1e-10 * scale
+
+    d = R.take(np.float64, n)
+    assert np.array_equal(d, np.diag(X0))
+    assert np.array_equal(R.mat(n), np.diag(d))
+    assert not R.mat(n).any()
+    assert R.o == len(R.b)
