@@ -177,6 +177,15 @@ int madam_tuned_layout(int n, int active);
  * solve the same system and the solutions agree to the PCG tolerance. */
 int madam_set_cg_lds_cap(long bytes, int one_pass);
 
+/* Testing hook, no reference counterpart: capacities of the pinned staging
+ * buffers through which a batch of nodes' small host-device transfers move
+ * as one copy each way (uploads, default 64 MiB; downloads, default
+ * 32 MiB; 0 keeps a default), so that tests can make them wrap and split
+ * on small problems.  Frees the current buffers (synchronising the
+ * stream); the next transfer allocates the new size.  Host memory only:
+ * results do not depend on the sizes. */
+int madam_set_staging(long up_bytes, long down_bytes);
+
 /* Launch mode of the ADMM iterations (no reference counterpart: the
  * reference runs one CPU loop, admm.c:520-704).  on = 1: the iterations
  * between two host checks of the stop flags are captured into a HIP graph

@@ -128,6 +128,13 @@ def set_cg_lds_cap(nbytes, one_pass=True):
     _check(lib().madam_set_cg_lds_cap(C.c_long(nbytes), C.c_int(1 if one_pass else 0)))
 
 
+def set_staging(up_bytes=0, down_bytes=0):
+    """Capacities of the pinned staging buffers of batched host-device
+    transfers (0: the defaults, 64 and 32 MiB); small values make them wrap
+    and split (testing)."""
+    _check(lib().madam_set_staging(C.c_long(up_bytes), C.c_long(down_bytes)))
+
+
 def set_graphs(on=True):
     """Replay recurring check intervals of the ADMM iterations from a
     captured HIP graph, or (on=False, the library's default) launch every

@@ -136,13 +136,16 @@ void store_fracsol(Run& r, int N, const std::vector<double>& lastcol) {
 int fetch_fracsol(std::vector<Run*>& runs, int N) {
   Engine& E = *g_eng;
   std::vector<std::vector<double>> cols(runs.size());
+  DownBatch db(E.stream);
+  int rc;
   for (size_t q = 0; q < runs.size(); ++q) {
     Run* r = runs[q];
     cols[q].resize(r->n);
-    int rc = down(cols[q].data(), E.slots[r->slot].X.p + (size_t)(r->n - 1) * r->n, r->n, E.stream);
-    if (rc) return rc;
+    if ((rc = db.get(cols[q].data(), E.slots[r->slot].X.p + (size_t)(r->n - 1) * r->n,
+                     (size_t)r->n)))
+      return rc;
   }
-  HIP_TRY(hipStreamSynchronize(E.stream));
+  if ((rc = db.finish())) return rc;
   for (size_t q = 0; q < runs.size(); ++q) store_fracsol(*runs[q], N, cols[q]);
   return MADAM_OK;
 }
@@ -182,13 +185,15 @@ int heuristic(madam_bab_ctx* c, std::vector<Run*>& runs, bool zero_planes) {
   // every run's best value and cut in one round trip
   std::vector<double> bvs(k);
   std::vector<std::vector<signed char>> zs(k);
+  DownBatch db(st);
   for (int q = 0; q < k; ++q) {
     Slot& sl = E.slots[runs[q]->slot];
     zs[q].resize(runs[q]->n);
-    if ((rc = down(&bvs[q], sl.gwbest.p, 1, st)) || (rc = down(zs[q].data(), sl.gwbz.p, runs[q]->n, st)))
+    if ((rc = db.get(&bvs[q], sl.gwbest.p, 1)) ||
+        (rc = db.get(zs[q].data(), sl.gwbz.p, (size_t)runs[q]->n)))
       return rc;
   }
-  HIP_TRY(hipStreamSynchronize(st));
+  if ((rc = db.finish())) return rc;
   std::vector<int> sol(c->N);
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
@@ -420,6 +425,8 @@ int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
     build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
                  (int)r.hept.size(), r.hept.data(), cs[q]);
   });
+  // every node's cut structure, duals and clears: one copy and one launch
+  UpBatch ub(st);
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
     if (bad[q] &&
@@ -433,7 +440,7 @@ int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
       if ((rc = up(sl.s.p, cs[q].y0.data(), cs[q].m, st)) ||
           (rc = up(sl.t.p, cs[q].y0.data(), cs[q].m, st)))
         return rc;
-      HIP_TRY(hipMemsetAsync(sl.u.p, 0, cs[q].m * sizeof(double), st));
+      if ((rc = zero(sl.u.p, cs[q].m * sizeof(double), st))) return rc;
     }
     if (r.capture && c->last_call && (rc = capture_call(c, r, sl, cs[q]))) return rc;
     NodeView v = E.hviews[r.slot];
@@ -450,7 +457,7 @@ int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
     F.hv.push_back(fresh);
     F.m.push_back(cs[q].m);
   }
-  return MADAM_OK;
+  return ub.flush();
 }
 
 // Advance the calls in flight until at least min_done of them (all, with
@@ -487,6 +494,7 @@ int advance_calls(madam_bab_ctx* c, Flight& F, int min_done, std::vector<Run*>& 
     }
   }
   if ((rc = admm_finish(hd, o, nd, ndm))) return rc;
+  DownBatch db(st);
   for (size_t q = 0; q < done.size(); ++q) {
     Run& r = *done[q];
     E.hviews[r.slot] = hd[q];
@@ -498,9 +506,9 @@ int advance_calls(madam_bab_ctx* c, Flight& F, int min_done, std::vector<Run*>& 
     c->ns_fallbacks += hd[q].ns_fallbacks;
     c->admm_calls += 1;
     r.s.resize(md[q]);
-    if (md[q] > 0 && (rc = down(r.s.data(), E.slots[r.slot].s.p, md[q], st))) return rc;
+    if ((rc = db.get(r.s.data(), E.slots[r.slot].s.p, (size_t)md[q]))) return rc;
   }
-  HIP_TRY(hipStreamSynchronize(st));
+  if ((rc = db.finish())) return rc;
   F.runs.swap(rest.runs);
   F.hv.swap(rest.hv);
   F.m.swap(rest.m);
@@ -521,6 +529,7 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
   std::vector<const int*> xf, sx;
   std::vector<double*> dst;
   std::vector<int> dq;
+  UpBatch ub(st);  // host-built subproblems: one copy and one launch
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
     Slot& sl = E.slots[r.slot];
@@ -540,6 +549,7 @@ int ipm_round(madam_bab_ctx* c, std::vector<Run*>& runs) {
     }
     nmax = std::max(nmax, r.n);
   }
+  if ((rc = ub.flush())) return rc;
   if (!dq.empty()) {
     std::vector<int> nd(dq.size());
     std::vector<double> nrm(dq.size());

@@ -1439,6 +1439,32 @@ static int chunks_for(int work, int per) {
   return c;
 }
 
+// Batched transfer pieces (staging buffer <-> node buffers): 16-byte
+// vector copies where both ends are 16-byte aligned, bytes otherwise.
+__global__ __launch_bounds__(256) void k_xfer(const XDesc* __restrict__ d, int nd) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  for (int q = blockIdx.x; q < nd; q += gridDim.x) {
+    const XDesc x = d[q];
+    const size_t nb = (size_t)x.bytes;
+    size_t done = 0;
+    if ((((uintptr_t)x.src | (uintptr_t)x.dst) & 15) == 0) {
+      const size_t n16 = nb >> 4;
+      u4* o = (u4*)x.dst;
+      const u4* in = (const u4*)x.src;
+      for (size_t i = threadIdx.x; i < n16; i += blockDim.x)
+        o[i] = in ? in[i] : u4{0u, 0u, 0u, 0u};
+      done = n16 << 4;
+    }
+    for (size_t i = done + threadIdx.x; i < nb; i += blockDim.x)
+      x.dst[i] = x.src ? x.src[i] : (char)0;
+  }
+}
+
+void launch_xfer(const XDesc* d, int nd, hipStream_t st) {
+  if (nd <= 0) return;
+  k_xfer<<<std::min(nd, 4096), 256, 0, st>>>(d, nd);
+}
+
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   // one cut per thread: the per-cut gathers are latency chains
   dim3 grid(chunks_for(mmax, WGS), nb);

@@ -513,45 +513,251 @@ inline madam_opts default_opts() {
 // copy) and the DMA runs asynchronously on the stream.  When the ring is
 // full the stream is synchronised before its start is reused; uploads of
 // more than a quarter of the ring go directly from pageable memory.
+//
+// Batched transfers.  The host work of a batch of nodes moves tens of
+// small arrays per node (cut structures, duals, flags, separation lists);
+// one copy each costs a dispatch on the stream, and a download into
+// pageable memory blocks the host until it lands.  Inside an UpBatch scope
+// up() and zero() only stage their data in the ring and record the piece;
+// the scope's flush moves the staged bytes in one DMA to a device mirror of
+// the ring and one k_xfer launch scatters them to their buffers.  Work that
+// reads those buffers must be issued after the flush.  A DownBatch gathers
+// its pieces on the device into a staging buffer (one k_xfer launch),
+// copies that to pinned memory in one DMA, synchronises the stream and
+// hands the pieces out on the host.
 struct PinnedRing {
   char* p = nullptr;
+  char* dev = nullptr;  // device mirror (batched uploads)
   size_t cap = 0, head = 0;
+  int batching = 0;
+  size_t base = 0;                    // start of the staged, not yet copied bytes
+  std::vector<madam::XDesc> pend;     // src: ring offset (or null: zero fill)
+  size_t want = kBytes;  // capacity of the next allocation
   static constexpr size_t kBytes = 64u << 20;
+  static constexpr size_t kZero = ~(size_t)0;
   void release() {
     if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = head = 0;
+    if (dev) (void)hipFree(dev);
+    p = dev = nullptr;
+    cap = head = base = 0;
+    batching = 0;
+    pend.clear();
   }
 };
 inline PinnedRing g_ring;
 
-template <class T>
-inline int up(T* dst, const T* src, size_t n, hipStream_t s) {
-  if (n == 0) return MADAM_OK;
-  const size_t bytes = n * sizeof(T);
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+inline void ring_ensure() {
   PinnedRing& R = g_ring;
-  if (!R.p && hipHostMalloc((void**)&R.p, PinnedRing::kBytes, hipHostMallocDefault) == hipSuccess)
-    R.cap = PinnedRing::kBytes;
+  if (!R.p && hipHostMalloc((void**)&R.p, R.want, hipHostMallocDefault) == hipSuccess)
+    R.cap = R.want;
+}
+
+// The staged pieces of the open batch: one DMA of [base, end) to the device
+// mirror (the descriptors are staged after the data), one scatter launch.
+inline int flush_up(hipStream_t s) {
+  PinnedRing& R = g_ring;
+  if (R.pend.empty()) return MADAM_OK;
+  if (!R.dev) HIP_TRY(hipMalloc((void**)&R.dev, R.cap));
+  const size_t doff = align256(R.head);
+  const size_t nd = R.pend.size();
+  if (doff + nd * sizeof(madam::XDesc) > R.cap) {
+    set_err("upload ring: %zu descriptors do not fit after offset %zu", nd, doff);
+    return MADAM_E_HIP;
+  }
+  madam::XDesc* hd = (madam::XDesc*)(R.p + doff);
+  for (size_t i = 0; i < nd; ++i) {
+    const size_t o = (size_t)(uintptr_t)R.pend[i].src;
+    hd[i].src = o == PinnedRing::kZero ? nullptr : R.dev + o;
+    hd[i].dst = R.pend[i].dst;
+    hd[i].bytes = R.pend[i].bytes;
+  }
+  const size_t end = doff + nd * sizeof(madam::XDesc);
+  HIP_TRY(hipMemcpyAsync(R.dev + R.base, R.p + R.base, end - R.base, hipMemcpyHostToDevice, s));
+  madam::launch_xfer((const madam::XDesc*)(R.dev + doff), (int)nd, s);
+  HIP_TRY(hipGetLastError());
+  R.pend.clear();
+  R.head = R.base = end;
+  return MADAM_OK;
+}
+
+// Make room in the ring for `bytes` of data at align256(head) plus the
+// descriptors of the open batch and one more piece; returns the offset.
+inline int ring_room(size_t bytes, hipStream_t s, size_t* off) {
+  PinnedRing& R = g_ring;
+  size_t o = align256(R.head);
+  const size_t desc = R.batching ? 256 + (R.pend.size() + 1) * sizeof(madam::XDesc) : 0;
+  if (o + bytes + desc > R.cap) {
+    int rc;
+    if ((rc = flush_up(s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));  // every copy out of the ring has finished
+    o = 0;
+    R.head = R.base = 0;
+  }
+  *off = o;
+  return MADAM_OK;
+}
+
+inline int up_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return MADAM_OK;
+  PinnedRing& R = g_ring;
+  ring_ensure();
   if (!R.p || bytes > R.cap / 4) {
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return MADAM_OK;
   }
-  size_t off = (R.head + 255) & ~(size_t)255;
-  if (off + bytes > R.cap) {
-    HIP_TRY(hipStreamSynchronize(s));  // every copy out of the ring has finished
-    off = 0;
-  }
+  size_t off;
+  int rc;
+  if ((rc = ring_room(bytes, s, &off))) return rc;
   memcpy(R.p + off, src, bytes);
-  HIP_TRY(hipMemcpyAsync(dst, R.p + off, bytes, hipMemcpyHostToDevice, s));
   R.head = off + bytes;
+  if (R.batching)
+    R.pend.push_back({(const char*)(uintptr_t)off, (char*)dst, (unsigned long long)bytes});
+  else
+    HIP_TRY(hipMemcpyAsync(dst, R.p + off, bytes, hipMemcpyHostToDevice, s));
   return MADAM_OK;
 }
+
+template <class T>
+inline int up(T* dst, const T* src, size_t n, hipStream_t s) {
+  return up_bytes((void*)dst, (const void*)src, n * sizeof(T), s);
+}
+
+// dst[0 .. bytes) = 0 on the stream (a piece of the open batch, if any)
+inline int zero(void* dst, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return MADAM_OK;
+  PinnedRing& R = g_ring;
+  if (!R.batching) {
+    HIP_TRY(hipMemsetAsync(dst, 0, bytes, s));
+    return MADAM_OK;
+  }
+  size_t off;
+  int rc;
+  if ((rc = ring_room(0, s, &off))) return rc;
+  R.pend.push_back({(const char*)(uintptr_t)PinnedRing::kZero, (char*)dst,
+                    (unsigned long long)bytes});
+  return MADAM_OK;
+}
+
+struct UpBatch {
+  hipStream_t s;
+  bool on = false;
+  explicit UpBatch(hipStream_t st) : s(st) {
+    ring_ensure();
+    PinnedRing& R = g_ring;
+    if (R.p && !R.batching) {  // (an inner scope joins the outer batch)
+      R.batching = 1;
+      R.base = R.head;
+      on = true;
+    }
+  }
+  int flush() {
+    if (!on) return MADAM_OK;
+    on = false;
+    const int rc = flush_up(s);
+    g_ring.batching = 0;
+    return rc;
+  }
+  ~UpBatch() { (void)flush(); }
+  UpBatch(const UpBatch&) = delete;
+  UpBatch& operator=(const UpBatch&) = delete;
+};
+
 template <class T>
 inline int down(T* dst, const T* src, size_t n, hipStream_t s) {
   if (n == 0) return MADAM_OK;
   HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
   return MADAM_OK;
 }
+
+struct DownRing {
+  char* host = nullptr;          // pinned
+  char* dev = nullptr;           // device staging
+  madam::XDesc* hdesc = nullptr;  // pinned descriptors
+  madam::XDesc* ddesc = nullptr;
+  size_t cap = 0;
+  size_t want = kBytes;  // capacity of the next allocation
+  static constexpr size_t kBytes = 32u << 20;
+  static constexpr size_t kPieces = 1u << 16;
+  bool ensure() {
+    if (cap) return true;
+    if (hipHostMalloc((void**)&host, want, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&hdesc, kPieces * sizeof(madam::XDesc), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void**)&dev, want) != hipSuccess ||
+        hipMalloc((void**)&ddesc, kPieces * sizeof(madam::XDesc)) != hipSuccess) {
+      (void)hipGetLastError();
+      release();
+      return false;
+    }
+    cap = want;
+    return true;
+  }
+  void release() {
+    if (host) (void)hipHostFree(host);
+    if (hdesc) (void)hipHostFree(hdesc);
+    if (dev) (void)hipFree(dev);
+    if (ddesc) (void)hipFree(ddesc);
+    host = dev = nullptr;
+    hdesc = ddesc = nullptr;
+    cap = 0;
+  }
+};
+inline DownRing g_dring;
+
+struct DownBatch {
+  hipStream_t s;
+  std::vector<void*> dst;
+  std::vector<const char*> src;
+  std::vector<size_t> bytes, off;
+  size_t used = 0;
+  explicit DownBatch(hipStream_t st) : s(st) {}
+  int add(void* hdst, const void* dsrc, size_t nbytes) {
+    if (nbytes == 0) return MADAM_OK;
+    if (!g_dring.ensure() || nbytes > g_dring.cap / 4) {
+      HIP_TRY(hipMemcpyAsync(hdst, dsrc, nbytes, hipMemcpyDeviceToHost, s));
+      return MADAM_OK;
+    }
+    size_t o = align256(used);
+    if (o + nbytes > g_dring.cap || dst.size() >= DownRing::kPieces) {
+      int rc;
+      if ((rc = finish())) return rc;
+      o = 0;
+    }
+    dst.push_back(hdst);
+    src.push_back((const char*)dsrc);
+    bytes.push_back(nbytes);
+    off.push_back(o);
+    used = o + nbytes;
+    return MADAM_OK;
+  }
+  template <class T>
+  int get(T* hdst, const T* dsrc, size_t n) {
+    return add((void*)hdst, (const void*)dsrc, n * sizeof(T));
+  }
+  // every piece on the host (and the stream synchronised)
+  int finish() {
+    const size_t nd = dst.size();
+    if (nd > 0) {
+      for (size_t i = 0; i < nd; ++i)
+        g_dring.hdesc[i] = {src[i], g_dring.dev + off[i], (unsigned long long)bytes[i]};
+      HIP_TRY(hipMemcpyAsync(g_dring.ddesc, g_dring.hdesc, nd * sizeof(madam::XDesc),
+                             hipMemcpyHostToDevice, s));
+      madam::launch_xfer(g_dring.ddesc, (int)nd, s);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(g_dring.host, g_dring.dev, used, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < nd; ++i) memcpy(dst[i], g_dring.host + off[i], bytes[i]);
+    dst.clear();
+    src.clear();
+    bytes.clear();
+    off.clear();
+    used = 0;
+    return MADAM_OK;
+  }
+};
 
 // Fill view v from slot sl (device pointers) for a node of order n.
 inline void bind_view(NodeView& v, Slot& sl, int n, const CutSet& cs) {

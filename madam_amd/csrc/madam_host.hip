@@ -383,6 +383,7 @@ void madam_close(void) {
   release_bj();
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
+  g_dring.release();
   delete g_eng;
   g_eng = nullptr;
 }
@@ -629,12 +630,14 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
   if ((rc = E.sepviews.ensure(k))) return rc;
   std::vector<madam::SepView> hv(k);
   int nmax = 2;
+  UpBatch ub(st);  // the clears and the views: one copy and one launch
   for (int b = 0; b < k; ++b) {
     SepSlot& sp = E.sep[b];
     if ((rc = sp.ensure(std::max(K[b], 1)))) return rc;
-    HIP_TRY(hipMemsetAsync(sp.ctl.p, 0, madam::CTL_N * sizeof(uint32_t), st));
-    HIP_TRY(hipMemsetAsync(sp.hist.p, 0, madam::SEP_NBIN * sizeof(uint32_t), st));
-    HIP_TRY(hipMemsetAsync(sp.maxbits.p, 0, sizeof(unsigned long long), st));
+    if ((rc = zero(sp.ctl.p, madam::CTL_N * sizeof(uint32_t), st)) ||
+        (rc = zero(sp.hist.p, madam::SEP_NBIN * sizeof(uint32_t), st)) ||
+        (rc = zero(sp.maxbits.p, sizeof(unsigned long long), st)))
+      return rc;
     madam::SepView& v = hv[b];
     v.n = E.hviews[slots[b]].n;
     v.K = K[b];
@@ -650,7 +653,7 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
     v.fval = sp.fval.p;
     nmax = std::max(nmax, v.n);
   }
-  if ((rc = up(E.sepviews.p, hv.data(), k, st))) return rc;
+  if ((rc = up(E.sepviews.p, hv.data(), k, st)) || (rc = ub.flush())) return rc;
   TIMED(PH_TRISEP, st, madam::launch_tri_separate(E.sepviews.p, k, nmax, st));
   HIP_TRY(hipGetLastError());
   lists.assign(k, {});
@@ -658,13 +661,14 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
   // two round trips for the whole batch: the counts, then the lists
   std::vector<uint32_t> ctl((size_t)k * madam::CTL_N);
   std::vector<unsigned long long> mbs(k);
+  DownBatch db(st);
   for (int b = 0; b < k; ++b) {
     SepSlot& sp = E.sep[b];
-    if ((rc = down(ctl.data() + (size_t)b * madam::CTL_N, sp.ctl.p, madam::CTL_N, st)) ||
-        (rc = down(&mbs[b], sp.maxbits.p, 1, st)))
+    if ((rc = db.get(ctl.data() + (size_t)b * madam::CTL_N, sp.ctl.p, madam::CTL_N)) ||
+        (rc = db.get(&mbs[b], sp.maxbits.p, 1)))
       return rc;
   }
-  HIP_TRY(hipStreamSynchronize(st));
+  if ((rc = db.finish())) return rc;
   // a boundary bin too full to rank (many equal or crowded values): the
   // exact refinement for the flagged nodes, then their counts again
   bool over = false;
@@ -673,9 +677,9 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
     TIMED(PH_TRISEP, st, madam::launch_tri_refine(E.sepviews.p, k, nmax, st));
     HIP_TRY(hipGetLastError());
     for (int b = 0; b < k; ++b)
-      if ((rc = down(ctl.data() + (size_t)b * madam::CTL_N, E.sep[b].ctl.p, madam::CTL_N, st)))
+      if ((rc = db.get(ctl.data() + (size_t)b * madam::CTL_N, E.sep[b].ctl.p, madam::CTL_N)))
         return rc;
-    HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = db.finish())) return rc;
     for (int b = 0; b < k; ++b)
       if (ctl[(size_t)b * madam::CTL_N + madam::CTL_OVER] == 1) {
         set_err("triangle separation: boundary refinement did not finish");
@@ -689,10 +693,10 @@ int mh::tri_separate_slots(const std::vector<int>& slots, const int* K, double t
     const uint32_t nf = ctl[(size_t)b * madam::CTL_N + madam::CTL_NFINAL];
     keys[b].resize(nf);
     vals[b].resize(nf);
-    if ((rc = down(keys[b].data(), sp.fkey.p, nf, st)) || (rc = down(vals[b].data(), sp.fval.p, nf, st)))
+    if ((rc = db.get(keys[b].data(), sp.fkey.p, nf)) || (rc = db.get(vals[b].data(), sp.fval.p, nf)))
       return rc;
   }
-  HIP_TRY(hipStreamSynchronize(st));
+  if ((rc = db.finish())) return rc;
   for (int b = 0; b < k; ++b) {
     const unsigned long long mb = mbs[b];
     const uint32_t nf = (uint32_t)keys[b].size();
@@ -904,6 +908,25 @@ int madam_set_cg_lds_cap(long bytes, int one_pass) {
   }
   g_cg_lds_cap = (size_t)bytes;
   g_cg_one_pass = one_pass ? 1 : 0;
+  return MADAM_OK;
+}
+
+int madam_set_staging(long up_bytes, long down_bytes) {
+  if (up_bytes < 0 || down_bytes < 0 || (up_bytes && up_bytes < 4096) ||
+      (down_bytes && down_bytes < 4096)) {
+    set_err("madam_set_staging: sizes must be 0 or at least 4096 bytes (%ld, %ld)", up_bytes,
+            down_bytes);
+    return MADAM_E_ARG;
+  }
+  if (g_ring.batching) {
+    set_err("madam_set_staging: a batched upload is open");
+    return MADAM_E_ARG;
+  }
+  if (g_eng) HIP_TRY(hipStreamSynchronize(g_eng->stream));
+  g_ring.release();
+  g_dring.release();
+  g_ring.want = up_bytes ? (size_t)up_bytes : PinnedRing::kBytes;
+  g_dring.want = down_bytes ? (size_t)down_bytes : DownRing::kBytes;
   return MADAM_OK;
 }
 

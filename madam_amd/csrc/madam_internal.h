@@ -129,4 +129,14 @@ void launch_mfma_test(const float* A32, const float* B32, float* C32, int np, co
                       const double* B64, double* C64, int n, double alpha, double beta,
                       hipStream_t stream);
 
+// One piece of a batched host-device transfer: `bytes` from src to dst,
+// both device addresses (src null: fill dst with zeros).
+struct XDesc {
+  const char* src;
+  char* dst;
+  unsigned long long bytes;
+};
+// Every piece of d[0..nd) (a device array), one workgroup per piece.
+void launch_xfer(const XDesc* d, int nd, hipStream_t stream);
+
 }  // namespace madam
