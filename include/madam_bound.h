@@ -263,7 +263,9 @@ int madam_tri_separate_noalloc(int n, const double *X, int K, double thresh, mad
  * independent runs on X: run r uses the sign pattern of types[r] (H = ee^T
  * with the first types[r]-1 entries of e negated) and its own random stream
  * (seed, stream_ids[r]); cost[r] and perms[r*k .. r*k+k-1] are the best
- * value and k-tuple, draws[r] the number of random values consumed. */
+ * value and k-tuple, draws[r] the number of random values consumed.  Reads
+ * X's lower triangle only (the reference copies it over the upper triangle
+ * first, qap_simulated_annealing.c:14-17; X itself is not modified). */
 int madam_qap_sa(int n, const double *X, int k, int nruns, const int *types,
                  const long long *stream_ids, unsigned long long seed, double *cost, int *perms,
                  long long *draws);

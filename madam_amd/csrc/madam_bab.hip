@@ -296,7 +296,7 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
       J.stream_id = c->stream_ctr++;
     }
   if ((rc = up(E.sajobs.p, jobs.data(), total, st))) return rc;
-  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, k, per, nmax, KK, E.saout.p, E.lds_limit, st));
+  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, k, per, nmax, KK, E.saout.p, E.lds_limit, E.cu_count, st));
   HIP_TRY(hipGetLastError());
   std::vector<madam::SaOut> out(total);
   if ((rc = down(out.data(), E.saout.p, total, st))) return rc;

@@ -774,7 +774,7 @@ int madam_qap_sa(int n, const double* X, int k, int nruns, const int* types,
     jobs[r].stream_id = stream_ids[r];
   }
   if ((rc = up(E.sajobs.p, jobs.data(), nruns, st))) return rc;
-  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, 1, nruns, n, k, E.saout.p, E.lds_limit, st));
+  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, 1, nruns, n, k, E.saout.p, E.lds_limit, E.cu_count, st));
   HIP_TRY(hipGetLastError());
   std::vector<madam::SaOut> out(nruns);
   if ((rc = down(out.data(), E.saout.p, nruns, st))) return rc;

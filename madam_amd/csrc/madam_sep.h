@@ -56,7 +56,7 @@ struct SaOut {
 };
 // jobs: nnodes groups of `per` runs, the runs of a group on one X
 void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, int kk, SaOut* out,
-                   size_t lds_limit, hipStream_t st);
+                   size_t lds_limit, int cu_count, hipStream_t st);
 void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipStream_t st);
 
 }  // namespace madam

@@ -390,9 +390,13 @@ __device__ __forceinline__ int rand31(uint64_t base, uint64_t& ctr) {
 }
 
 // Jobs are laid out node by node, `per` runs each; grid (ceil(per / SA_WG),
-// nodes), so that a workgroup's runs share one X.  With XL the workgroup
+// nodes), so that a workgroup's runs share one X.  XM = 1: the workgroup
 // first copies X into LDS (n^2 doubles after the permutation buffers) and
 // every run reads it there: same values, same trajectory, LDS latency.
+// XM = 2 (orders whose X fits LDS only by half): the lower triangle only,
+// row by row (entry (a, b), a >= b, at a (a + 1) / 2 + b).  Every mode reads
+// entry (a, b) from the lower triangle, as the reference does after copying
+// X's lower triangle over its upper (qap_simulated_annealing.c:14-17).
 // x mod n for 0 <= x < 2^31 by a double-precision reciprocal and one
 // correction (the quotient estimate is off by at most one): exact, and
 // cheaper than the integer division of a run-time divisor.
@@ -406,9 +410,20 @@ __device__ __forceinline__ int mod_n(int x, int n, double inv_n) {
   return r;
 }
 
+// entry (a, b) of a symmetric matrix from its lower triangle: packed row
+// by row, or in place in the column-major n x n matrix
+__device__ __forceinline__ double xp_at(const double* __restrict__ P, int a, int b) {
+  const int hi = a > b ? a : b, lo = a > b ? b : a;
+  return P[((hi * (hi + 1)) >> 1) + lo];
+}
+__device__ __forceinline__ double x_low(const double* __restrict__ X, int n, int a, int b) {
+  const int hi = a > b ? a : b, lo = a > b ? b : a;
+  return X[(size_t)hi + (size_t)lo * n];
+}
+
 // K: the cut order of every run of the launch (5 or 7; 0 = read from the
 // job), so that the loops over the permutation's first K entries unroll.
-template <bool XL, int K>
+template <int XM, int K>
 __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, SaOut* out) {
 #pragma clang fp contract(off)
   extern __shared__ unsigned short permbuf[];
@@ -417,12 +432,22 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
   const int first = blockIdx.y * per + blockIdx.x * SA_WG;  // a valid job of this workgroup
   const int nst = jobs[first].nstride;
   const double* __restrict__ X = jobs[first].X;
-  if (XL) {
+  const double* __restrict__ Xp = nullptr;  // XM = 2: packed lower triangle
+  if (XM == 1) {
     const int nx = jobs[first].n;
     double* Xl = (double*)(permbuf + (((size_t)SA_WG * nst + 3) & ~(size_t)3));
     for (int f = threadIdx.x; f < nx * nx; f += SA_WG) Xl[f] = X[f];
     __syncthreads();
     X = Xl;
+  } else if (XM == 2) {
+    const int nx = jobs[first].n;
+    double* Xl = (double*)(permbuf + (((size_t)SA_WG * nst + 3) & ~(size_t)3));
+    for (int f = threadIdx.x; f < nx * nx; f += SA_WG) {
+      const int b = f / nx, a = f - b * nx;  // X[f] = entry (a, b)
+      if (a >= b) Xl[((a * (a + 1)) >> 1) + b] = X[f];
+    }
+    __syncthreads();
+    Xp = Xl;
   }
   if (rr0 >= per) return;
   const SaJob J = jobs[r];
@@ -433,7 +458,7 @@ __global__ __launch_bounds__(SA_WG) void k_qap_sa(const SaJob* jobs, int per, Sa
   uint64_t ctr = 0;
   // H = e e^T with e_a = -1 for a < neg (cutting_planes.c:242-244, 438-441)
 #define SG(a) ((a) < J.neg ? -1 : 1)
-#define XV(a, b) X[(size_t)(a) + (size_t)(b) * n]
+#define XV(a, b) (XM == 2 ? xp_at(Xp, a, b) : x_low(X, n, a, b))
   const double sumX = *J.sumX;
   double temp0 = (double)(k * k) * sumX / (double)(n * (n - 1));
   double cost = 1e9;
@@ -555,29 +580,40 @@ void launch_sumx(const double* const* X, const int* n, double* out, int nb, hipS
   k_sumx<<<nb, SUMX_WG, 0, st>>>(X, n, out);
 }
 
+// Layout: X in LDS (whole, else its lower triangle) where that still lets
+// every workgroup of the launch be resident at once -- each is one wave of
+// serial chains, and a CU that runs several of them (one per SIMD) does
+// more than one LDS-fed wave; else X is read from global memory.
+// pw05_150.0 proof, 1024 nodes in flight: 207 s with X in LDS wherever it
+// fits whole, 214 s with the lower triangle in LDS as well, 201 s (annealing
+// phase 34 -> 28 s) with this rule.
 void launch_qap_sa(const SaJob* jobs, int nnodes, int per, int nmax, int kk, SaOut* out,
-                   size_t lds_limit, hipStream_t st) {
+                   size_t lds_limit, int cu_count, hipStream_t st) {
   const dim3 grid((per + SA_WG - 1) / SA_WG, nnodes);
   const size_t lperm = (((size_t)SA_WG * sa_nstride(nmax) + 3) & ~(size_t)3) * sizeof(unsigned short);
   const size_t lx = (size_t)nmax * nmax * sizeof(double);
-  const bool xl = lperm + lx <= lds_limit;
-  const size_t lds = xl ? lperm + lx : lperm;
-#define MADAM_SA(XLV, KV) k_qap_sa<XLV, KV><<<grid, SA_WG, lds, st>>>(jobs, per, out)
-  if (xl) {
-    if (kk == 5)
-      MADAM_SA(true, 5);
-    else if (kk == 7)
-      MADAM_SA(true, 7);
-    else
-      MADAM_SA(true, 0);
+  const size_t lxp = (size_t)nmax * (nmax + 1) / 2 * sizeof(double);
+  const long wgs = (long)grid.x * grid.y;
+  const long per_cu = std::max(1L, (wgs + cu_count - 1) / std::max(1, cu_count));
+  const size_t room = lds_limit / (size_t)std::min(per_cu, 8L);
+  const int xm = lperm + lx <= room ? 1 : (lperm + lxp <= room ? 2 : 0);
+  const size_t lds = lperm + (xm == 1 ? lx : (xm == 2 ? lxp : 0));
+#define MADAM_SA(XMV, KV) k_qap_sa<XMV, KV><<<grid, SA_WG, lds, st>>>(jobs, per, out)
+#define MADAM_SA_K(XMV)   \
+  if (kk == 5)            \
+    MADAM_SA(XMV, 5);     \
+  else if (kk == 7)       \
+    MADAM_SA(XMV, 7);     \
+  else                    \
+    MADAM_SA(XMV, 0);
+  if (xm == 1) {
+    MADAM_SA_K(1)
+  } else if (xm == 2) {
+    MADAM_SA_K(2)
   } else {
-    if (kk == 5)
-      MADAM_SA(false, 5);
-    else if (kk == 7)
-      MADAM_SA(false, 7);
-    else
-      MADAM_SA(false, 0);
+    MADAM_SA_K(0)
   }
+#undef MADAM_SA_K
 #undef MADAM_SA
 }
 

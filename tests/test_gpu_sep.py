@@ -86,11 +86,20 @@ def test_triangle_separation_boundary_overflow(kind, K):
         assert G.tri_separate(X, K, 1e-3)[1] == lst_g
 
 
-# n = 15 and 60 read X from its copy in LDS, n = 140 and 200 from HBM
-# (an n x n FP64 copy does not fit the 160 KB of LDS of a workgroup)
-@pytest.mark.parametrize("n,k,seed", [(15, 5, 11), (60, 5, 12), (60, 7, 13), (140, 7, 14), (200, 5, 15)])
-def test_qap_sa_same_stream_as_oracle(n, k, seed):
+# n = 15 and 60 read X from its copy in LDS, n = 140 from its lower
+# triangle in LDS (an n x n FP64 copy does not fit the 160 KB of LDS of a
+# workgroup, half of it does), n = 200 from HBM; "asym": X with its upper
+# triangle perturbed, which every layout must ignore (the reference copies
+# the lower triangle over the upper first, qap_simulated_annealing.c:14-17)
+@pytest.mark.parametrize("n,k,seed,asym", [(15, 5, 11, False), (60, 5, 12, False),
+                                           (60, 7, 13, False), (140, 7, 14, False),
+                                           (200, 5, 15, False), (60, 5, 16, True),
+                                           (140, 5, 17, True), (200, 7, 18, True)])
+def test_qap_sa_same_stream_as_oracle(n, k, seed, asym):
     X = psd_X(n, seed)
+    if asym:
+        noise = np.random.default_rng(seed).standard_normal((n, n))
+        X = np.asfortranarray(X + 1e-3 * np.triu(noise, 1))
     ntypes = 3 if k == 5 else 4
     types = [1 + (r % ntypes) for r in range(2 * ntypes)]
     sids = list(range(100, 100 + len(types)))
