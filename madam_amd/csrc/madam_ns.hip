@@ -234,6 +234,81 @@ __device__ __forceinline__ void gemm_block2_f32(PA A, PB B, PC Cm, int np, float
                      ((acc[0][x] + acc[1][x]) + acc[2][x]) + acc[3][x]);
 }
 
+// The same block with its results kept in registers (the k loop of
+// gemm_block2_f32, same bits), stored later by block2_store.
+struct Blk2 {
+  f32x4 c[4];
+  bool on[4];
+  int bi, bj;
+};
+
+template <class PA, class PB>
+__device__ __forceinline__ void block2_compute(PA A, PB B, int np, float alpha, float beta, int t,
+                                               Blk2& out) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const int nt = np >> 4;
+  int bi, bj;
+  tile_decode(t, bi, bj);
+  bi >>= 4;
+  bj >>= 4;
+  bool vr[2], vc[2];
+  int ra[2], cb[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    vr[u] = 2 * bi + u < nt;
+    vc[u] = 2 * bj + u < nt;
+    ra[u] = (vr[u] ? (2 * bi + u) * 16 : 0) + r;  // a missing tile row reads row 0, unused
+    cb[u] = (vc[u] ? (2 * bj + u) * 16 : 0) + r;
+  }
+  bool on[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    on[x] = vr[x >> 1] && vc[x & 1] && 2 * bj + (x & 1) <= 2 * bi + (x >> 1);
+  const int kq = np >> 2;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) acc[u][x] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < kq; s += 4) {
+    float a[4][2], b[4][2];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = u * kq + s + q;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        a[u][v] = A[ra[v] + (size_t)k * np];
+        float bb = beta * B[cb[v] + (size_t)k * np];
+        if (cb[v] == k) bb += alpha;
+        b[u][v] = bb;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        if (on[x])
+          acc[u][x] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][x >> 1], b[u][x & 1], acc[u][x], 0,
+                                                           0, 0);
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    out.on[x] = on[x];
+    out.c[x] = ((acc[0][x] + acc[1][x]) + acc[2][x]) + acc[3][x];
+  }
+  out.bi = bi;
+  out.bj = bj;
+}
+
+
+template <class PC>
+__device__ __forceinline__ void block2_store(PC Cm, int np, const Blk2& b) {
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+    if (b.on[x]) store_tile_f32(Cm, np, (2 * b.bi + (x >> 1)) * 16, (2 * b.bj + (x & 1)) * 16, b.c[x]);
+}
+
 template <class PA, class PB, class PC>
 __device__ __forceinline__ void sym_gemm_f32_b2(PA A, PB B, PC Cm, int np, float alpha,
                                                 float beta) {
@@ -350,10 +425,11 @@ __device__ __forceinline__ void sym_gemm_f64(const double* __restrict__ A, const
 // MODE 1: the three FP32 iterates live in LDS (3 n_p^2 floats, n_p <=
 // 112); MODE 2: one n_p^2 LDS buffer (n_p <= 192) holding the matrix both
 // operands of the next product read -- S for T = S^2, T for U, U for S' --
-// refilled from the node's ns32 scratch by a coalesced copy after each
-// product, so only S' = S (a I + U) reads one operand from L2/HBM; MODE 0:
-// all three in ns32.  The products and their bits are the same in every
-// mode.
+// and S also in the node's ns32 scratch, from where S' = S (a I + U) reads
+// it.  A product's results stay in registers until every wave has
+// finished reading the buffer, then overwrite it (S' also goes to ns32):
+// no round trip through L2/HBM for T and U.  MODE 0: all three in ns32.
+// The products and their bits are the same in every mode.
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore_done) {
   extern __shared__ float sm32[];
@@ -365,7 +441,7 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   float* Sb = MODE == 1 ? sm32 : nd.ns32;
   float* Tb = Sb + npp;
   float* Ub = Tb + npp;
-  float* Lb = sm32;  // MODE 2 staging buffer
+  float* Lb = sm32;  // MODE 2, 3: the LDS operand buffer
   const double* __restrict__ Wm = nd.W;
   double fv[1] = {0.0};
   for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
@@ -376,22 +452,44 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
     const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
     const float v = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
     Sb[f] = v;
-    if (MODE == 2) Lb[f] = v;
+    if (MODE >= 2) Lb[f] = v;
   }
   __syncthreads();
-  // global -> LDS copy of one n_p x n_p matrix (n_p^2 is a multiple of 256)
+  float* S = Sb;
+  float* T = Tb;
+  float* U = Ub;
+  // MODE 2: C = A (alpha I + beta B) into the LDS buffer (and into G when
+  // given); each wave holds one 2 x 2 block (n_p <= 160: at most 15
+  // blocks for the 16 waves)
+  auto product_lds = [&](const float* A, float alpha, float beta, float* G) {
+    const int wave = tid >> 6;
+    const int ng = ((np >> 4) + 1) >> 1;
+    const bool mine = wave < ng * (ng + 1) / 2;
+    Blk2 res;
+    if (mine) block2_compute(A, Lb, np, alpha, beta, wave, res);
+    __syncthreads();  // every wave has read the buffer
+    if (mine) {
+      block2_store(Lb, np, res);
+      if (G) block2_store(G, np, res);
+    }
+    __syncthreads();
+  };
+  // MODE 3: the LDS buffer refilled from ns32 by a coalesced copy after
+  // each product (n_p 176..192, where a wave would hold two blocks)
   auto stage = [&](const float* G) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     for (size_t f = (size_t)tid * 4; f < npp; f += (size_t)WG * 4)
       *(f4*)(Lb + f) = *(const f4*)(G + f);
     __syncthreads();
   };
-  float* S = Sb;
-  float* T = Tb;
-  float* U = Ub;
   for (int s = 0; s < NS_STEPS; ++s) {
     const float a = NS_COEF[s][0], b = NS_COEF[s][1], c = NS_COEF[s][2];
     if (MODE == 2) {
+      product_lds(Lb, 0.f, 1.f, nullptr);  // T = S^2
+      product_lds(Lb, b, c, nullptr);      // U = T (b I + c T)
+      product_lds(Sb, a, 1.f, Sb);         // S' = S (a I + U), S from ns32
+      continue;
+    } else if (MODE == 3) {
       sym_gemm_f32_b2(Lb, Lb, T, np, 0.f, 1.f);  // T = S^2 (S staged)
       __syncthreads();
       stage(T);
@@ -463,8 +561,10 @@ void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_
   const size_t lds = ns_lds_bytes(nmax);
   if (lds + 1024 <= lds_limit)
     k_ns_project<1><<<nb, 1024, lds, st>>>(nodes, ignore_done);
-  else if (lds / 3 + 1024 <= lds_limit)
+  else if (lds / 3 + 1024 <= lds_limit && ((nmax + 15) & ~15) <= 160)
     k_ns_project<2><<<nb, 1024, lds / 3, st>>>(nodes, ignore_done);
+  else if (lds / 3 + 1024 <= lds_limit)
+    k_ns_project<3><<<nb, 1024, lds / 3, st>>>(nodes, ignore_done);
   else
     k_ns_project<0><<<nb, 1024, 0, st>>>(nodes, ignore_done);
 }

@@ -119,10 +119,12 @@ def _layout_graph(spec):
 
 
 # orders 60 and 100: one-workgroup kernel with the iterates in LDS; 120, 140
-# and 150: one staged LDS buffer; 200: iterates in HBM
+# and 150: one LDS operand buffer that each product's results overwrite;
+# 180: one LDS buffer refilled from HBM; 200: iterates in HBM
 @pytest.mark.parametrize("graph,kgon", [("g05_60.0", False), ("w01_100.3", True),
                                         ("gnp120", False), ("gnp140", False),
-                                        ("gnp150", False), ("gnp200", False)])
+                                        ("gnp150", False), ("gnp180", False),
+                                        ("gnp200", False)])
 def test_tile_parallel_layout_is_bit_identical(graph, kgon):
     """The tile-parallel launch sequences (16 x 16 tiles, and 2 x 2 blocks of
     tiles) compute the same tiles with the same instruction order as the
