@@ -54,6 +54,10 @@ __device__ inline void tile_decode(int t, int& i0, int& j0) {
   j0 = tj << 4;
 }
 
+// Leading dimension of the one-workgroup kernel's LDS operand buffer
+// (mode 2): n_p, or n_p + 16 where n_p is 0 or 32 (mod 64).
+__host__ __device__ inline int ns_lds_ld(int np) { return (np & 63) == 0 || (np & 63) == 32 ? np + 16 : np; }
+
 // Tile products split K into four contiguous quarters, each accumulated
 // by its own MFMA chain and summed as ((q0 + q1) + q2) + q3: one wave can
 // run the four chains interleaved (independent accumulators keep the MFMA
@@ -243,8 +247,8 @@ struct Blk2 {
 };
 
 template <class PA, class PB>
-__device__ __forceinline__ void block2_compute(PA A, PB B, int np, float alpha, float beta, int t,
-                                               Blk2& out) {
+__device__ __forceinline__ void block2_compute(PA A, int lda, PB B, int ldb, int np, float alpha,
+                                               float beta, int t, Blk2& out) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const int nt = np >> 4;
@@ -278,8 +282,8 @@ __device__ __forceinline__ void block2_compute(PA A, PB B, int np, float alpha, 
       const int k = u * kq + s + q;
 #pragma unroll
       for (int v = 0; v < 2; ++v) {
-        a[u][v] = A[ra[v] + (size_t)k * np];
-        float bb = beta * B[cb[v] + (size_t)k * np];
+        a[u][v] = A[ra[v] + (size_t)k * lda];
+        float bb = beta * B[cb[v] + (size_t)k * ldb];
         if (cb[v] == k) bb += alpha;
         b[u][v] = bb;
       }
@@ -303,10 +307,10 @@ __device__ __forceinline__ void block2_compute(PA A, PB B, int np, float alpha, 
 
 
 template <class PC>
-__device__ __forceinline__ void block2_store(PC Cm, int np, const Blk2& b) {
+__device__ __forceinline__ void block2_store(PC Cm, int ldc, const Blk2& b) {
 #pragma unroll
   for (int x = 0; x < 4; ++x)
-    if (b.on[x]) store_tile_f32(Cm, np, (2 * b.bi + (x >> 1)) * 16, (2 * b.bj + (x & 1)) * 16, b.c[x]);
+    if (b.on[x]) store_tile_f32(Cm, ldc, (2 * b.bi + (x >> 1)) * 16, (2 * b.bj + (x & 1)) * 16, b.c[x]);
 }
 
 template <class PA, class PB, class PC>
@@ -442,6 +446,9 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
   float* Tb = Sb + npp;
   float* Ub = Tb + npp;
   float* Lb = sm32;  // MODE 2, 3: the LDS operand buffer
+  // MODE 2: its leading dimension, = 16 or 48 (mod 64) so that the four
+  // k rows an MFMA operand load spans fall in different banks
+  const int ldl = MODE == 2 ? ns_lds_ld(np) : np;
   const double* __restrict__ Wm = nd.W;
   double fv[1] = {0.0};
   for (int f = tid; f < n * n; f += WG) fv[0] += Wm[f] * Wm[f];
@@ -452,7 +459,7 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
     const int j = (int)(f / np), i = (int)(f - (size_t)j * np);
     const float v = (i < n && j < n) ? (float)Wm[i + (size_t)j * n] * inv : 0.f;
     Sb[f] = v;
-    if (MODE >= 2) Lb[f] = v;
+    if (MODE >= 2) Lb[i + (size_t)j * ldl] = v;
   }
   __syncthreads();
   float* S = Sb;
@@ -466,10 +473,10 @@ __global__ __launch_bounds__(1024) void k_ns_project(NodeView* nodes, int ignore
     const int ng = ((np >> 4) + 1) >> 1;
     const bool mine = wave < ng * (ng + 1) / 2;
     Blk2 res;
-    if (mine) block2_compute(A, Lb, np, alpha, beta, wave, res);
+    if (mine) block2_compute(A, A == Lb ? ldl : np, Lb, ldl, np, alpha, beta, wave, res);
     __syncthreads();  // every wave has read the buffer
     if (mine) {
-      block2_store(Lb, np, res);
+      block2_store(Lb, ldl, res);
       if (G) block2_store(G, np, res);
     }
     __syncthreads();
@@ -561,8 +568,12 @@ void launch_ns_project(NodeView* nodes, int nb, int nmax, int ignore_done, size_
   const size_t lds = ns_lds_bytes(nmax);
   if (lds + 1024 <= lds_limit)
     k_ns_project<1><<<nb, 1024, lds, st>>>(nodes, ignore_done);
-  else if (lds / 3 + 1024 <= lds_limit && ((nmax + 15) & ~15) <= 160)
-    k_ns_project<2><<<nb, 1024, lds / 3, st>>>(nodes, ignore_done);
+  else if (lds / 3 + 1024 <= lds_limit && ((nmax + 15) & ~15) <= 160) {
+    // (every node's buffer, n_p x ns_lds_ld(n_p), fits the largest's)
+    const size_t np = (size_t)((nmax + 15) & ~15);
+    const size_t l2 = np * (np + 16) * sizeof(float);
+    k_ns_project<2><<<nb, 1024, l2, st>>>(nodes, ignore_done);
+  }
   else if (lds / 3 + 1024 <= lds_limit)
     k_ns_project<3><<<nb, 1024, lds / 3, st>>>(nodes, ignore_done);
   else
