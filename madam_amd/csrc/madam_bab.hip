@@ -425,8 +425,9 @@ int begin_calls(madam_bab_ctx* c, std::vector<Run*>& runs, Flight& F) {
     build_cutset(r.n, (int)r.tri.size(), r.tri.data(), (int)r.pent.size(), r.pent.data(),
                  (int)r.hept.size(), r.hept.data(), cs[q]);
   });
-  // every node's cut structure, duals and clears: one copy and one launch
-  UpBatch ub(st);
+  // every node's cut structure, duals and clears: one copy and one launch;
+  // cs outlives the batch, so the copies into the ring run on host threads
+  UpBatch ub(st, true);
   for (int q = 0; q < k; ++q) {
     Run& r = *runs[q];
     if (bad[q] &&

@@ -1465,6 +1465,17 @@ void launch_xfer(const XDesc* d, int nd, hipStream_t st) {
   k_xfer<<<std::min(nd, 4096), 256, 0, st>>>(d, nd);
 }
 
+__global__ __launch_bounds__(256) void k_p2a(const P2aJob* __restrict__ jobs) {
+  const P2aJob J = jobs[blockIdx.x];
+  for (int f = threadIdx.x; f < J.nn; f += blockDim.x) J.p2a[f] = -1;
+  __syncthreads();
+  for (int a = threadIdx.x; a < J.nact; a += blockDim.x) J.p2a[J.apair[a]] = a;
+}
+
+void launch_p2a(const P2aJob* jobs, int njobs, hipStream_t st) {
+  if (njobs > 0) k_p2a<<<njobs, 256, 0, st>>>(jobs);
+}
+
 void launch_admm_prep(NodeView* nodes, int nb, int mmax, hipStream_t st) {
   // one cut per thread: the per-cut gathers are latency chains
   dim3 grid(chunks_for(mmax, WGS), nb);

@@ -532,6 +532,16 @@ struct PinnedRing {
   int batching = 0;
   size_t base = 0;                    // start of the staged, not yet copied bytes
   std::vector<madam::XDesc> pend;     // src: ring offset (or null: zero fill)
+  std::vector<madam::P2aJob> jobs;    // pair maps to build after the scatter
+  // copies into the ring left for the flush (UpBatch with deferred copies:
+  // the sources live until then), made on host threads together
+  struct HostCopy {
+    char* dst;
+    const void* src;
+    size_t bytes;
+  };
+  int defer_copies = 0;
+  std::vector<HostCopy> hcopies;
   size_t want = kBytes;  // capacity of the next allocation
   static constexpr size_t kBytes = 64u << 20;
   static constexpr size_t kZero = ~(size_t)0;
@@ -541,7 +551,10 @@ struct PinnedRing {
     p = dev = nullptr;
     cap = head = base = 0;
     batching = 0;
+    defer_copies = 0;
     pend.clear();
+    jobs.clear();
+    hcopies.clear();
   }
 };
 inline PinnedRing g_ring;
@@ -554,16 +567,33 @@ inline void ring_ensure() {
     R.cap = R.want;
 }
 
+// Ring bytes the open batch needs after its data: the descriptors of its
+// pieces and its pair-map jobs, with one more of each.
+inline size_t ring_tail(const PinnedRing& R) {
+  return 512 + (R.pend.size() + 1) * sizeof(madam::XDesc) +
+         (R.jobs.size() + 1) * sizeof(madam::P2aJob);
+}
+
 // The staged pieces of the open batch: one DMA of [base, end) to the device
-// mirror (the descriptors are staged after the data), one scatter launch.
+// mirror (the descriptors and pair-map jobs are staged after the data), one
+// scatter launch, then one launch building the pair maps.
 inline int flush_up(hipStream_t s) {
   PinnedRing& R = g_ring;
-  if (R.pend.empty()) return MADAM_OK;
+  if (!R.hcopies.empty()) {
+    parallel_for((int)R.hcopies.size(), [&](int i) {
+      const PinnedRing::HostCopy& h = R.hcopies[i];
+      memcpy(h.dst, h.src, h.bytes);
+    });
+    R.hcopies.clear();
+  }
+  if (R.pend.empty() && R.jobs.empty()) return MADAM_OK;
   if (!R.dev) HIP_TRY(hipMalloc((void**)&R.dev, R.cap));
   const size_t doff = align256(R.head);
-  const size_t nd = R.pend.size();
-  if (doff + nd * sizeof(madam::XDesc) > R.cap) {
-    set_err("upload ring: %zu descriptors do not fit after offset %zu", nd, doff);
+  const size_t nd = R.pend.size(), nj = R.jobs.size();
+  const size_t joff = align256(doff + nd * sizeof(madam::XDesc));
+  const size_t end = joff + nj * sizeof(madam::P2aJob);
+  if (end > R.cap) {  // ring_room keeps this room
+    set_err("upload ring: %zu descriptors and %zu jobs do not fit after offset %zu", nd, nj, doff);
     return MADAM_E_HIP;
   }
   madam::XDesc* hd = (madam::XDesc*)(R.p + doff);
@@ -573,21 +603,23 @@ inline int flush_up(hipStream_t s) {
     hd[i].dst = R.pend[i].dst;
     hd[i].bytes = R.pend[i].bytes;
   }
-  const size_t end = doff + nd * sizeof(madam::XDesc);
+  if (nj) memcpy(R.p + joff, R.jobs.data(), nj * sizeof(madam::P2aJob));
   HIP_TRY(hipMemcpyAsync(R.dev + R.base, R.p + R.base, end - R.base, hipMemcpyHostToDevice, s));
   madam::launch_xfer((const madam::XDesc*)(R.dev + doff), (int)nd, s);
+  madam::launch_p2a((const madam::P2aJob*)(R.dev + joff), (int)nj, s);
   HIP_TRY(hipGetLastError());
   R.pend.clear();
+  R.jobs.clear();
   R.head = R.base = end;
   return MADAM_OK;
 }
 
-// Make room in the ring for `bytes` of data at align256(head) plus the
-// descriptors of the open batch and one more piece; returns the offset.
+// Make room in the ring for `bytes` of data at align256(head) plus what
+// the open batch needs after it; returns the offset.
 inline int ring_room(size_t bytes, hipStream_t s, size_t* off) {
   PinnedRing& R = g_ring;
   size_t o = align256(R.head);
-  const size_t desc = R.batching ? 256 + (R.pend.size() + 1) * sizeof(madam::XDesc) : 0;
+  const size_t desc = R.batching ? ring_tail(R) : 0;
   if (o + bytes + desc > R.cap) {
     int rc;
     if ((rc = flush_up(s))) return rc;
@@ -610,7 +642,10 @@ inline int up_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
   size_t off;
   int rc;
   if ((rc = ring_room(bytes, s, &off))) return rc;
-  memcpy(R.p + off, src, bytes);
+  if (R.batching && R.defer_copies)
+    R.hcopies.push_back({R.p + off, src, bytes});
+  else
+    memcpy(R.p + off, src, bytes);
   R.head = off + bytes;
   if (R.batching)
     R.pend.push_back({(const char*)(uintptr_t)off, (char*)dst, (unsigned long long)bytes});
@@ -640,14 +675,17 @@ inline int zero(void* dst, size_t bytes, hipStream_t s) {
   return MADAM_OK;
 }
 
+// defer: the sources of up() stay valid until the flush, which copies
+// them into the ring on host threads (else each up() copies at once)
 struct UpBatch {
   hipStream_t s;
   bool on = false;
-  explicit UpBatch(hipStream_t st) : s(st) {
+  explicit UpBatch(hipStream_t st, bool defer = false) : s(st) {
     ring_ensure();
     PinnedRing& R = g_ring;
     if (R.p && !R.batching) {  // (an inner scope joins the outer batch)
       R.batching = 1;
+      R.defer_copies = defer ? 1 : 0;
       R.base = R.head;
       on = true;
     }
@@ -657,12 +695,35 @@ struct UpBatch {
     on = false;
     const int rc = flush_up(s);
     g_ring.batching = 0;
+    g_ring.defer_copies = 0;
+    g_ring.hcopies.clear();
     return rc;
   }
   ~UpBatch() { (void)flush(); }
   UpBatch(const UpBatch&) = delete;
   UpBatch& operator=(const UpBatch&) = delete;
 };
+
+// A cut structure's pair map (n^2 ints, -1 but at the active pairs), built
+// on the device from the uploaded apair after the batch's scatter instead of
+// being uploaded (host_p2a: the host's copy, uploaded only when there is no
+// pinned ring).
+inline int p2a_job(int* p2a, const int* apair, size_t nn, int nact, const int* host_p2a,
+                   hipStream_t s) {
+  PinnedRing& R = g_ring;
+  ring_ensure();
+  if (!R.p) return up(p2a, host_p2a, nn, s);
+  if (!R.batching) {
+    UpBatch ub(s);
+    int rc = p2a_job(p2a, apair, nn, nact, host_p2a, s);
+    return rc ? rc : ub.flush();
+  }
+  size_t off;
+  int rc;
+  if ((rc = ring_room(0, s, &off))) return rc;
+  R.jobs.push_back({p2a, apair, (int)nn, nact});
+  return MADAM_OK;
+}
 
 template <class T>
 inline int down(T* dst, const T* src, size_t n, hipStream_t s) {
@@ -823,7 +884,7 @@ inline int upload_cuts(Slot& sl, const CutSet& cs, hipStream_t s) {
       (rc = up(sl.aent16.p, cs.aent16.data(), cs.aent16.size(), s)) ||
       (rc = up(sl.cact16.p, cs.cact16.data(), cs.cact16.size(), s)) ||
       (rc = up(sl.apair.p, cs.apair.data(), cs.apair.size(), s)) ||
-      (rc = up(sl.p2a.p, cs.p2a.data(), cs.p2a.size(), s)) ||
+      (rc = p2a_job(sl.p2a.p, sl.apair.p, cs.p2a.size(), cs.nact, cs.p2a.data(), s)) ||
       (rc = up(sl.cscale.p, cs.cscale.data(), cs.cscale.size(), s)))
     return rc;
   if (!cs.tcut.empty() &&

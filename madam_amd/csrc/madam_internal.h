@@ -138,5 +138,14 @@ struct XDesc {
 };
 // Every piece of d[0..nd) (a device array), one workgroup per piece.
 void launch_xfer(const XDesc* d, int nd, hipStream_t stream);
+// The pair map of a cut structure built on the device: p2a[0 .. nn) = -1,
+// then p2a[apair[a]] = a for a < nact (each active pair's lower entry ->
+// its index), one workgroup per job.
+struct P2aJob {
+  int* p2a;
+  const int* apair;
+  int nn, nact;
+};
+void launch_p2a(const P2aJob* jobs, int njobs, hipStream_t stream);
 
 }  // namespace madam
