@@ -13,16 +13,20 @@ ranks fail on any tree that branches; the workers share the one GPU).
       madam_amd/dropin/madam_admm.c in place of admm.c and ipm_mc_pk.c:
       ADMM_solver (admm.h:212) and ipm_mc_pk (admm.h:269) run on the GPU
       with the reference's prototypes and globals.
-  oracle/_ref/madam_dropin_ops   as madam_dropin_admm, with
-      madam_amd/dropin/madam_operators.c and madam_evaluate.c in place of
-      operators.c, qap_simulated_annealing.c and evaluate.c: op_B, op_Bt,
-      project_SDP, the separation's annealing (qap_simulated_annealing,
-      admm.h:286) and createSubproblem/getFixedValue (admm.h:246-247) run
-      through the library too.
+  oracle/_ref/madam_dropin_ops   the reference's bounding.c (the
+      cutting-plane loop's control flow) with every routine it calls
+      through the library: madam_admm.c, madam_operators.c,
+      madam_cutting_planes.c, madam_heuristic.c and madam_evaluate.c in
+      place of admm.c, ipm_mc_pk.c, operators.c,
+      qap_simulated_annealing.c, cutting_planes.c, heuristic.c and
+      evaluate.c.
   oracle/_ref/madam_ops_check    tests/dropin_ops_check.c: calls op_B,
-      op_Bt, project_SDP, diag, Diag and op_Diag of madam_operators.c
-      through the reference's prototypes and globals (admm.h:272-277,
-      global_var.h); checked against the oracle below.
+      op_Bt, project_SDP, diag, Diag and op_Diag of madam_operators.c, and
+      the update*/getViolated_* separation entries of
+      madam_cutting_planes.c and mc_1opt/GW_heuristic/update_best of
+      madam_heuristic.c, through the reference's prototypes and globals
+      (admm.h:237-277, global_var.h); checked against the oracle and the
+      library below.
 
 All are built by `make -C oracle all` where the reference tree exists.
 Instances: g05_60.0 (rudy, params file values; its root prunes: optimum
@@ -181,7 +185,7 @@ def test_reference_operator_prototypes(tmp_path, n, seed):
     X0, tri, pent, hept, t = _cut_case(n, seed)
     _write_case(tmp_path / "in.bin", X0, tri, pent, hept, t)
     print("madam_ops_check", n, seed, flush=True)
-    r = subprocess.run([OPS_CHECK, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+    r = subprocess.run([OPS_CHECK, "ops", str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
                        capture_output=True, text=True, timeout=120)
     print(r.stdout[-1000:], r.stderr[-1000:])
     assert r.returncode == 0
@@ -217,4 +221,159 @@ def test_reference_operator_prototypes(tmp_path, n, seed):
     assert np.array_equal(d, np.diag(X0))
     assert np.array_equal(R.mat(n), np.diag(d))
     assert not R.mat(n).any()
+    assert R.o == len(R.b)
+
+
+def _glibc_rands(seed, count):
+    libc = C.CDLL("libc.so.6")
+    libc.srand(C.c_uint(seed))
+    return [int(libc.rand()) for _ in range(count)]
+
+
+def _kgon_top_k(cost, perms, types, K):
+    """The top-K rules of getViolated_Pentagonal/HeptagonalInequalities
+    (cutting_planes.c:247-303) over the runs in order: the first K violated
+    runs fill the list, a later run below the tracked value replaces the
+    tracked slot, and after a replacement the rescan (which looks for a value
+    above BIG_NUMBER) leaves the slot where it is with the value BIG_NUMBER."""
+    lst, slot, slotval = [], 0, -1e9
+    for r, v in enumerate(cost):
+        if not v < 1.0:
+            continue
+        entry = (int(types[r]), tuple(int(a) for a in perms[r]), float(v))
+        if len(lst) < K:
+            if not lst or v > slotval:
+                slotval, slot = v, len(lst)
+            lst.append(entry)
+        elif v < slotval:
+            lst[slot] = entry
+            slotval = 1e9
+    return lst
+
+
+@pytest.mark.skipif(not os.path.exists(OPS_CHECK), reason="oracle/_ref/madam_ops_check not built")
+@pytest.mark.parametrize("n,seed", [(40, 7), (70, 8)])
+def test_reference_separation_and_rounding_prototypes(tmp_path, n, seed):
+    """updateTriangle/Pentagonal/HeptagonalInequalities, mc_1opt and
+    GW_heuristic (cutting_planes.c:132-605, heuristic.c:24-223) with the
+    reference's prototypes and globals:
+      triangles  survivors (dual >= 1e-5) in order, then the oracle's
+                 separation rows not among them, in enumeration order; the
+                 maximum row value bit-exact
+      k-gons     survivors, then the list the reference's top-K rules build
+                 from the library's annealing runs (each seed is the first
+                 glibc rand() after the driver's srand), duplicates dropped
+      mc_1opt    the oracle's 1-opt: same cut and value, bit-exact
+      GW         the best of the library's rounding cuts by the test's
+                 evaluateSolution, in hyperplane order"""
+    from madam_amd import instances as I
+    import madam_amd as G
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((n, 5))
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    Xs = V @ V.T
+    np.fill_diagonal(Xs, 1.0)
+    X0 = np.asfortranarray(np.tril(Xs) + np.triu(rng.standard_normal((n, n)), 1))  # upper ignored
+    mx_o, lst_o = O.tri_separate(Xs, 200, 1e-3)
+    key = lambda c: (c[0], c[1], c[2], c[3])
+    sep_sorted = sorted(lst_o, key=key)
+    tri = [(c[0], c[1], c[2], c[3]) for c in sep_sorted[:5]]  # will be found again
+    for _ in range(25):
+        i, j, k = sorted(rng.choice(n, 3, replace=False), reverse=True)
+        tri.append((int(rng.integers(1, 5)), int(i), int(j), int(k)))
+    pent = [(int(rng.integers(1, 4)), tuple(int(v) for v in rng.permutation(n)[:5])) for _ in range(10)]
+    hept = [(int(rng.integers(1, 5)), tuple(int(v) for v in rng.permutation(n)[:7])) for _ in range(6)]
+    m = len(tri) + len(pent) + len(hept)
+    yy = np.where(rng.random(m) < 0.5, 0.0, rng.random(m) + 1e-3)
+    yy[:5] = 1.0
+    L = np.asfortranarray(I.objective_matrix(*I.gnp(n, 0.5, [-2, -1, 1, 2], seed)))
+    Mz = 5
+    Z = np.asfortranarray(rng.standard_normal((n, Mz)))
+    x0 = rng.choice([-1, 1], size=n).astype(np.int32)
+    K3, K5, K7, T5, T7, num, srand_seed = 200, 30, 30, 12, 10, 40, 1234 + seed
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(np.array([n, len(tri), len(pent), len(hept), K3, K5, K7, T5, T7, Mz, num,
+                          srand_seed], np.int32).tobytes())
+        f.write(np.array([1e-3]).tobytes())
+        f.write(X0.tobytes(order="F"))
+        for ty, i, j, k in tri:
+            f.write(np.array([i, j, k, ty], np.int32).tobytes())
+        for ty, perm in pent:
+            f.write(np.array([ty, *perm], np.int32).tobytes())
+        for ty, perm in hept:
+            f.write(np.array([ty, *perm], np.int32).tobytes())
+        f.write(yy.tobytes())
+        f.write(L.tobytes(order="F"))
+        f.write(Z.tobytes(order="F"))
+        f.write(x0.tobytes())
+    print("madam_ops_check sep", n, seed, flush=True)
+    r = subprocess.run([OPS_CHECK, "sep", str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    R = _Reader(tmp_path / "out.bin")
+    s5, s7, s_gw = [_glibc_rands(srand_seed + q, 1)[0] for q in (1, 2, 3)]
+
+    # triangles
+    v = R.take(np.float64, 1)[0]
+    cnt, added, sub = R.take(np.int32, 3)
+    got = []
+    for _ in range(cnt):
+        e = R.take(np.int32, 4)
+        got.append((int(e[3]), int(e[0]), int(e[1]), int(e[2]), float(R.take(np.float64, 1)[0])))
+    nt = len(tri)
+    surv = [(ty, i, j, k, float(yy[c])) for c, (ty, i, j, k) in enumerate(tri) if yy[c] >= 1e-5]
+    have = {s_[:4] for s_ in surv}
+    new = [(c[0], c[1], c[2], c[3], 0.0) for c in sep_sorted if key(c) not in have]
+    assert v == mx_o
+    assert sub == nt - len(surv) and added == len(new)
+    assert got == surv + new
+
+    # pentagons and heptagons
+    Xsym = np.asfortranarray(Xs)
+    for k, items, off, K, T, ntypes, sd in ((5, pent, nt, K5, T5, 3, s5),
+                                            (7, hept, nt + len(pent), K7, T7, 4, s7)):
+        v = R.take(np.float64, 1)[0]
+        cnt, added, sub = R.take(np.int32, 3)
+        got = []
+        for _ in range(cnt):
+            ty = int(R.take(np.int32, 1)[0])
+            perm = tuple(int(a) for a in R.take(np.int32, k))
+            val, yv = R.take(np.float64, 2)
+            got.append((ty, perm, float(yv)))
+        types = [1 + q % ntypes for q in range(T * ntypes)]
+        cost, perms, _ = G.qap_sa(Xsym, k, types, list(range(len(types))), seed=sd)
+        lst = _kgon_top_k(cost, perms, types, K)
+        surv = [(ty, perm, float(yy[off + c])) for c, (ty, perm) in enumerate(items)
+                if yy[off + c] >= 1e-5]
+        have = {(ty, perm) for ty, perm, _ in surv}
+        new = [(ty, perm, 0.0) for ty, perm, _ in lst if (ty, perm) not in have]
+        print(k, "runs", len(types), "violated", int((cost < 1).sum()), "list", len(lst),
+              "added", len(new))
+        assert v == cost.min()
+        assert sub == len(items) - len(surv) and added == len(new)
+        assert got == surv + new
+
+    # mc_1opt
+    cost = R.take(np.float64, 1)[0]
+    x1 = R.take(np.int32, n)
+    c_o, x_o = O.one_opt(x0, L)
+    assert cost == c_o
+    assert np.array_equal(x1, x_o)
+
+    # GW_heuristic
+    best = R.take(np.float64, 1)[0]
+    xb = R.take(np.int32, n - 1)
+    _, cuts = G.gw_rounding(L, Z, num, seed=s_gw, stream=0)
+    e_best, e_x = -1e9, None
+    for h in range(num):
+        sol = [(int(c) + 1) // 2 for c in cuts[h, :n - 1]]
+        val = 0.0
+        for i in range(n - 1):
+            for j in range(n - 1):
+                val += L[j, i] * sol[i] * sol[j]
+        if e_best < val:
+            e_best, e_x = val, sol
+    assert best == e_best
+    assert list(xb) == e_x
     assert R.o == len(R.b)
