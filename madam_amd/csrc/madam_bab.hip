@@ -254,68 +254,100 @@ int separate_tri(madam_bab_ctx* c, std::vector<Run*>& runs) {
   return MADAM_OK;
 }
 
-int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
-                  const std::vector<int>& offsets) {
+// One round of pentagonal (hept = false) or heptagonal separation for a
+// set of runs: the annealing runs of every run in one launch, then the
+// top-K list and the cut set's purge and append on host threads.  In two
+// parts so that two rounds (the pentagonal runs and the heptagonal runs
+// known beforehand) can run side by side on the two streams.
+struct KgonRound {
+  std::vector<Run*> runs;
+  std::vector<int> offsets;
+  bool hept = false;
+  int KK = 5, ntypes = 3, per = 0, total = 0;
+  std::vector<int> types;
+  std::vector<madam::SaOut> out;
+  madam::SaOut* dout = nullptr;
+};
+
+// Uploads (on the main stream) and the launch on `run_on`; with run_on !=
+// the main stream the caller orders it after the uploads.
+// Stream ids id0, id0 + 1, ... in job order.
+int kgon_launch(madam_bab_ctx* c, KgonRound& K, bool second, hipStream_t run_on, long long id0) {
   Engine& E = *g_eng;
   const hipStream_t st = E.stream;
-  const int k = (int)runs.size();
-  if (k == 0) return MADAM_OK;
-  const int KK = hept ? 7 : 5, ntypes = hept ? 4 : 3;
-  const int trials = hept ? c->params.Hepta_Trials : c->params.Pent_Trials;
-  const int per = trials * ntypes;
-  const int total = per * k;
-  if (total == 0) return MADAM_OK;
+  const int k = (int)K.runs.size();
+  K.KK = K.hept ? 7 : 5;
+  K.ntypes = K.hept ? 4 : 3;
+  const int trials = K.hept ? c->params.Hepta_Trials : c->params.Pent_Trials;
+  K.per = trials * K.ntypes;
+  K.total = K.per * k;
+  if (K.total == 0) return MADAM_OK;
+  DBuf<double>& sumx = second ? E.sumx2 : E.sumx;
+  DBuf<const double*>& xptrs = second ? E.xptrs2 : E.xptrs;
+  DBuf<int>& nvals = second ? E.nvals2 : E.nvals;
+  DBuf<madam::SaJob>& sajobs = second ? E.sajobs2 : E.sajobs;
+  DBuf<madam::SaOut>& saout = second ? E.saout2 : E.saout;
   int rc;
-  if ((rc = E.sumx.ensure(k)) || (rc = E.xptrs.ensure(k)) || (rc = E.nvals.ensure(k)) ||
-      (rc = E.sajobs.ensure(total)) || (rc = E.saout.ensure(total)))
+  if ((rc = sumx.ensure(k)) || (rc = xptrs.ensure(k)) || (rc = nvals.ensure(k)) ||
+      (rc = sajobs.ensure(K.total)) || (rc = saout.ensure(K.total)))
     return rc;
   std::vector<const double*> xp(k);
   std::vector<int> ns(k);
   int nmax = 2;
   for (int q = 0; q < k; ++q) {
-    xp[q] = E.slots[runs[q]->slot].X.p;
-    ns[q] = runs[q]->n;
+    xp[q] = E.slots[K.runs[q]->slot].X.p;
+    ns[q] = K.runs[q]->n;
     nmax = std::max(nmax, ns[q]);
   }
-  if ((rc = up(E.xptrs.p, xp.data(), k, st)) || (rc = up(E.nvals.p, ns.data(), k, st))) return rc;
-  madam::launch_sumx(E.xptrs.p, E.nvals.p, E.sumx.p, k, st);
-  std::vector<madam::SaJob> jobs(total);
-  std::vector<int> types(per);
+  if ((rc = up(xptrs.p, xp.data(), k, st)) || (rc = up(nvals.p, ns.data(), k, st))) return rc;
+  madam::launch_sumx(xptrs.p, nvals.p, sumx.p, k, st);
+  std::vector<madam::SaJob> jobs(K.total);
+  K.types.assign(K.per, 0);
   for (int t = 0; t < trials; ++t)
-    for (int ty = 1; ty <= ntypes; ++ty) types[t * ntypes + ty - 1] = ty;
+    for (int ty = 1; ty <= K.ntypes; ++ty) K.types[t * K.ntypes + ty - 1] = ty;
   for (int q = 0; q < k; ++q)
-    for (int r = 0; r < per; ++r) {
-      madam::SaJob& J = jobs[(size_t)q * per + r];
+    for (int r = 0; r < K.per; ++r) {
+      madam::SaJob& J = jobs[(size_t)q * K.per + r];
       J.X = xp[q];
-      J.sumX = E.sumx.p + q;
+      J.sumX = sumx.p + q;
       J.n = ns[q];
-      J.k = KK;
-      J.neg = types[r] - 1;
+      J.k = K.KK;
+      J.neg = K.types[r] - 1;
       J.nstride = madam::sa_nstride(nmax);
       J.seed = c->seed;
-      J.stream_id = c->stream_ctr++;
+      J.stream_id = id0 + (long long)q * K.per + r;
     }
-  if ((rc = up(E.sajobs.p, jobs.data(), total, st))) return rc;
-  TIMED(PH_SA, st, madam::launch_qap_sa(E.sajobs.p, k, per, nmax, KK, E.saout.p, E.lds_limit, E.cu_count, st));
+  if ((rc = up(sajobs.p, jobs.data(), K.total, st))) return rc;
+  if (run_on != st) {
+    HIP_TRY(hipEventRecord(E.ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(run_on, E.ev_fork, 0));
+  }
+  TIMED(PH_SA, run_on,
+        madam::launch_qap_sa(sajobs.p, k, K.per, nmax, K.KK, saout.p, E.lds_limit, E.cu_count,
+                             run_on));
   HIP_TRY(hipGetLastError());
-  std::vector<madam::SaOut> out(total);
-  if ((rc = down(out.data(), E.saout.p, total, st))) return rc;
-  HIP_TRY(hipStreamSynchronize(st));
-  E.timer.flush();
+  K.dout = saout.p;
+  return MADAM_OK;
+}
+
+// The results (the caller has synchronised), list and cut-set upkeep.
+void kgon_finish(madam_bab_ctx* c, KgonRound& K) {
+  const int k = (int)K.runs.size();
+  if (K.total == 0) return;
   parallel_for(k, [&](int q) {
-    std::vector<double> cost(per);
-    std::vector<int> perms((size_t)per * KK);
-    Run& R = *runs[q];
-    for (int r = 0; r < per; ++r) {
-      const madam::SaOut& o = out[(size_t)q * per + r];
+    std::vector<double> cost(K.per);
+    std::vector<int> perms((size_t)K.per * K.KK);
+    Run& R = *K.runs[q];
+    for (int r = 0; r < K.per; ++r) {
+      const madam::SaOut& o = K.out[(size_t)q * K.per + r];
       cost[r] = o.cost;
-      for (int a = 0; a < KK; ++a) perms[(size_t)r * KK + a] = o.perm[a];
+      for (int a = 0; a < K.KK; ++a) perms[(size_t)r * K.KK + a] = o.perm[a];
     }
-    const double* dual = R.s.data() + offsets[q];
+    const double* dual = R.s.data() + K.offsets[q];
     double best;
-    if (hept) {
+    if (K.hept) {
       std::vector<madam_hept> list;
-      kgon_list<madam_hept, 7>(cost, perms, types, c->params.HeptaIneq, list, best);
+      kgon_list<madam_hept, 7>(cost, perms, K.types, c->params.HeptaIneq, list, best);
       purge_append(R.hept, dual, list, [](const madam_hept& a, const madam_hept& b) {
         if (a.type != b.type) return false;
         for (int x = 0; x < 7; ++x)
@@ -324,7 +356,7 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
       });
     } else {
       std::vector<madam_pent> list;
-      kgon_list<madam_pent, 5>(cost, perms, types, c->params.PentIneq, list, best);
+      kgon_list<madam_pent, 5>(cost, perms, K.types, c->params.PentIneq, list, best);
       purge_append(R.pent, dual, list, [](const madam_pent& a, const madam_pent& b) {
         if (a.type != b.type) return false;
         for (int x = 0; x < 5; ++x)
@@ -334,7 +366,53 @@ int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
       R.viol5 = best;
     }
   });
+}
+
+// Pentagonal separation of `ps` and, beside it on the second stream,
+// heptagonal separation of `hs` (the runs whose heptagonal round does not
+// depend on the pentagonal one's result); either may be empty.
+int separate_kgon_pair(madam_bab_ctx* c, std::vector<Run*>& ps, const std::vector<int>& poff,
+                       std::vector<Run*>& hs, const std::vector<int>& hoff) {
+  Engine& E = *g_eng;
+  const hipStream_t st = E.stream;
+  KgonRound P, H;
+  P.runs = ps;
+  P.offsets = poff;
+  H.runs = hs;
+  H.offsets = hoff;
+  H.hept = true;
+  int rc;
+  const bool both = !ps.empty() && !hs.empty();
+  // stream ids: the pentagonal runs', then the heptagonal runs'
+  const long long np_ids =
+      (long long)ps.size() * c->params.Pent_Trials * 3;
+  const long long id0 = c->stream_ctr;
+  if ((!hs.empty() && (rc = kgon_launch(c, H, both, both ? E.stream2 : st, id0 + np_ids))) ||
+      (!ps.empty() && (rc = kgon_launch(c, P, false, st, id0))))
+    return rc;
+  c->stream_ctr = id0 + P.total + H.total;
+  if (both && H.total > 0) {
+    HIP_TRY(hipEventRecord(E.ev_join, E.stream2));
+    HIP_TRY(hipStreamWaitEvent(st, E.ev_join, 0));
+  }
+  DownBatch db(st);
+  P.out.resize(P.total);
+  H.out.resize(H.total);
+  if ((rc = db.get(P.out.data(), P.dout, (size_t)P.total)) ||
+      (rc = db.get(H.out.data(), H.dout, (size_t)H.total)) || (rc = db.finish()))
+    return rc;
+  E.timer.flush();
+  kgon_finish(c, P);
+  kgon_finish(c, H);
   return MADAM_OK;
+}
+
+int separate_kgon(madam_bab_ctx* c, std::vector<Run*>& runs, bool hept,
+                  const std::vector<int>& offsets) {
+  std::vector<Run*> none;
+  const std::vector<int> noff;
+  return hept ? separate_kgon_pair(c, none, noff, runs, offsets)
+              : separate_kgon_pair(c, runs, offsets, none, noff);
 }
 
 // ADMM calls in flight: their runs and device views, the views kept up to
@@ -685,25 +763,32 @@ int end_calls(madam_bab_ctx* c, std::vector<Run*>& act) {
       penta[q] = (int)sep[q]->pent.size();
     }
     if ((rc = separate_tri(c, sep))) return rc;
-    std::vector<Run*> ps, hs;
-    std::vector<int> poff, hoff;
+    // pentagonal separation; heptagonal separation beside it for the runs
+    // past the pentagonal rounds, which take it whatever the pentagonal
+    // result, then for the runs that take it because of that result
+    std::vector<Run*> ps, hs_now, hs_after;
+    std::vector<int> poff, hoff_now, hoff_after;
     for (size_t q = 0; q < sep.size(); ++q) {
       Run* r = sep[q];
       if (P.include_Pent && (r->count >= P.triag_iter || r->viol3 < 0.2)) {
         ps.push_back(r);
         poff.push_back(triag[q]);
       }
-    }
-    if ((rc = separate_kgon(c, ps, false, poff))) return rc;
-    for (size_t q = 0; q < sep.size(); ++q) {
-      Run* r = sep[q];
-      if (P.include_Hepta && ((r->count >= P.triag_iter + P.pent_iter) ||
-                              (r->viol3 < 0.2 && (1 - r->viol5 < 0.4)))) {
-        hs.push_back(r);
-        hoff.push_back(triag[q] + penta[q]);
+      if (P.include_Hepta && r->count >= P.triag_iter + P.pent_iter) {
+        hs_now.push_back(r);
+        hoff_now.push_back(triag[q] + penta[q]);
       }
     }
-    if ((rc = separate_kgon(c, hs, true, hoff))) return rc;
+    if ((rc = separate_kgon_pair(c, ps, poff, hs_now, hoff_now))) return rc;
+    for (size_t q = 0; q < sep.size(); ++q) {
+      Run* r = sep[q];
+      if (P.include_Hepta && !(r->count >= P.triag_iter + P.pent_iter) &&
+          (r->viol3 < 0.2 && (1 - r->viol5 < 0.4))) {
+        hs_after.push_back(r);
+        hoff_after.push_back(triag[q] + penta[q]);
+      }
+    }
+    if ((rc = separate_kgon(c, hs_after, true, hoff_after))) return rc;
   }
   c->seconds_sep += now_s() - t0;
   for (Run* r : act) r->done = r->prune || r->giveup || (r->nbit >= P.ADMM_itermax);

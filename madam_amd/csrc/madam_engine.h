@@ -456,15 +456,19 @@ struct SepSlot {
 struct Engine {
   std::vector<SepSlot> sep;
   DBuf<madam::SepView> sepviews;
-  DBuf<madam::SaJob> sajobs;
+  DBuf<madam::SaJob> sajobs, sajobs2;  // (2: a second annealing launch in flight)
   DBuf<madam::GwNode> gwnodes;
-  DBuf<madam::SaOut> saout;
-  DBuf<double> sumx;
-  DBuf<const double*> xptrs;
-  DBuf<int> nvals;
+  DBuf<madam::SaOut> saout, saout2;
+  DBuf<double> sumx, sumx2;
+  DBuf<const double*> xptrs, xptrs2;
+  DBuf<int> nvals, nvals2;
   PhaseTimer timer;
   int device = -1;
   hipStream_t stream = nullptr;
+  // a second stream (default priority) for work that runs beside the
+  // main stream's, ordered against it by the two events
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   size_t lds_limit = 0;
   int cu_count = 256;
   std::vector<Slot> slots;

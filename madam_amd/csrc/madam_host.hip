@@ -351,9 +351,12 @@ int madam_open(int device) {
   e->device = device;
   e->lds_limit = prop.sharedMemPerBlock;
   e->cu_count = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete e;
-    set_err("hipStreamCreate failed");
+    set_err("hipStreamCreate / hipEventCreate failed");
     return MADAM_E_HIP;
   }
   g_eng = e;
@@ -367,6 +370,11 @@ void madam_close(void) {
   for (auto& s : g_eng->sep) s.release();
   g_eng->sepviews.release();
   g_eng->sajobs.release();
+  g_eng->sajobs2.release();
+  g_eng->saout2.release();
+  g_eng->sumx2.release();
+  g_eng->xptrs2.release();
+  g_eng->nvals2.release();
   g_eng->gwnodes.release();
   g_eng->saout.release();
   g_eng->sumx.release();
@@ -381,6 +389,10 @@ void madam_close(void) {
   release_safe_tiles();
   release_chunk_graph();
   release_bj();
+  (void)hipStreamSynchronize(g_eng->stream2);
+  (void)hipStreamDestroy(g_eng->stream2);
+  (void)hipEventDestroy(g_eng->ev_fork);
+  (void)hipEventDestroy(g_eng->ev_join);
   (void)hipStreamDestroy(g_eng->stream);
   g_ring.release();
   g_dring.release();
