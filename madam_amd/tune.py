@@ -18,9 +18,9 @@ import os
 import sys
 import time
 
-NPS = [16, 32, 48, 64, 80, 96, 112, 128, 160, 192, 224, 256, 320, 384, 448, 512, 640, 768,
-       896, 1024]
-NBS = [1, 2, 4, 8, 16, 32, 64, 128, 256]
+NPS = [16, 32, 48, 64, 80, 96, 112, 128, 144, 160, 176, 192, 224, 256, 320, 384, 448, 512, 640,
+       768, 896, 1024]
+NBS = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512]
 
 
 def cells(nps, nbs, max_elems=1 << 27):
